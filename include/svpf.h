@@ -1,0 +1,307 @@
+/*
+ * svpf.h -- C ABI of the MI355X Stein variational particle filter.
+ *
+ * Drop-in for the reference's public step surface (include/svpf.cuh): same
+ * function names, argument meaning and SVPFState field names, with
+ * hipStream_t in place of cudaStream_t.  Each declaration cites the reference
+ * declaration it replaces (paths relative to the reference repository root).
+ *
+ * Differences from the reference, all additive:
+ *   - errors are reported: every call records a status readable through
+ *     svpf_last_error(state); svpf_create returns NULL on any failure;
+ *   - features the reference offers but this build does not implement
+ *     (Student-t state noise, Heun integration, the two-factor branch) are
+ *     rejected with SVPF_ERR_UNSUPPORTED instead of being silently ignored;
+ *   - svpf_step_graph really replays a captured HIP graph per tick;
+ *   - three configuration fields: bandwidth_mode, use_linear_beta,
+ *     use_ksd_early_stop (defaults reproduce the reference).
+ */
+#ifndef SVPF_AMD_H
+#define SVPF_AMD_H
+
+#include <hip/hip_runtime_api.h>
+#include <math.h>
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* include/svpf.cuh:47-60 */
+#define SVPF_DEFAULT_PARTICLES 512
+#define SVPF_DEFAULT_STEIN_STEPS 5
+#define SVPF_DEFAULT_NU 5.0f
+#define SVPF_STEIN_STEP_SIZE 0.1f
+#define SVPF_H_MIN -15.0f
+#define SVPF_H_MAX 5.0f
+#define SVPF_SMOOTH_MAX_LAG 8
+
+/* include/svpf.cuh:84-90 */
+#define SVPF_STEIN_SIGN_LEGACY 0
+#define SVPF_STEIN_SIGN_PAPER 1
+#ifndef SVPF_STEIN_SIGN_DEFAULT
+#define SVPF_STEIN_SIGN_DEFAULT SVPF_STEIN_SIGN_LEGACY
+#endif
+
+/* Bandwidth rules (bandwidth_mode). */
+#define SVPF_BANDWIDTH_REFERENCE_EMA 0 /* src/svpf_opt_kernels.cu:1159-1235 */
+#define SVPF_BANDWIDTH_MEDIAN 1        /* ell^2 = lower median of pairwise r^2 */
+#define SVPF_BANDWIDTH_SILVERMAN 2     /* src/svpf_fully_fused_v2.cu:397-424 */
+
+/* Status codes (svpf_last_error). */
+#define SVPF_OK 0
+#define SVPF_ERR_INVALID_ARGUMENT 1
+#define SVPF_ERR_UNSUPPORTED 2
+#define SVPF_ERR_HIP 3
+#define SVPF_ERR_NOT_INITIALIZED 4
+#define SVPF_ERR_OUT_OF_MEMORY 5
+
+/* include/svpf.cuh:257-262 */
+typedef struct {
+    float rho;
+    float sigma_z;
+    float mu;
+    float gamma;
+} SVPFParams;
+
+struct svpf_impl;
+
+/* include/svpf.cuh:290-547.  Configuration is by direct field writes between
+ * calls, as in the reference; every field below is read on each tick. */
+typedef struct {
+    /* device arrays [n_particles] (include/svpf.cuh:292-305) */
+    float* h;
+    float* h_prev;
+    float* h_pred;
+    float* grad_log_p;
+    float* log_weights;
+    float* d_grad_v;
+
+    int n_particles;
+    int n_stein_steps;
+    float nu;
+    float student_t_const;
+    float student_t_implied_offset;
+    int timestep;
+    float y_prev;
+    hipStream_t stream;
+
+    float lik_offset;
+    int use_exact_gradient;
+
+    int use_svld;
+    int use_annealing;
+    int use_adaptive_beta;
+    int n_anneal_steps;
+    float temperature;
+    float rmsprop_rho;
+    float rmsprop_eps;
+
+    int use_mim;
+    float mim_jump_prob;
+    float mim_jump_scale;
+
+    int use_local_params;
+    float delta_rho;
+    float delta_sigma;
+
+    int use_asymmetric_rho;
+    float rho_up;
+    float rho_down;
+
+    int use_newton;
+    int use_full_newton;
+
+    int use_guided;
+    float guided_alpha_base;
+    float guided_alpha_shock;
+    float guided_innovation_threshold;
+
+    int use_rejuvenation;
+    float rejuv_ksd_threshold;
+    float rejuv_prob;
+    float rejuv_blend;
+
+    int use_guide;
+    int use_guide_preserving;
+    float guide_strength;
+    float guide_mean;
+    float guide_var;
+    float guide_K;
+    int guide_initialized;
+
+    int use_adaptive_guide;
+    float guide_strength_base;
+    float guide_strength_max;
+    float guide_innovation_threshold;
+    float vol_prev;
+
+    int use_adaptive_mu;
+    float mu_state;
+    float mu_var;
+    float mu_process_var;
+    float mu_obs_var_scale;
+    float mu_min;
+    float mu_max;
+
+    int use_adaptive_sigma;
+    float sigma_boost_threshold;
+    float sigma_boost_max;
+    float sigma_z_effective;
+
+    int stein_repulsive_sign;
+    int use_fan_mode;
+
+    int stein_min_steps;
+    int stein_max_steps;
+    float ksd_improvement_threshold;
+    float ksd_prev;
+    int stein_steps_used;
+
+    int use_student_t_state; /* rejected: SVPF_ERR_UNSUPPORTED */
+    float nu_state;
+
+    int use_smoothing;
+    int smooth_lag;
+    int smooth_output_lag;
+    int smooth_head;
+    float smooth_h_mean[SVPF_SMOOTH_MAX_LAG];
+    float smooth_h_var[SVPF_SMOOTH_MAX_LAG];
+    float smooth_y[SVPF_SMOOTH_MAX_LAG];
+
+    int use_persistent_kernel;
+    int persistent_kernel_supported;
+    int use_heun;      /* rejected: SVPF_ERR_UNSUPPORTED */
+    int use_two_factor; /* rejected: SVPF_ERR_UNSUPPORTED */
+
+    /* extensions (defaults reproduce the reference) */
+    int bandwidth_mode;
+    int use_linear_beta;
+    int use_ksd_early_stop;
+
+    struct svpf_impl* impl;
+} SVPFState;
+
+/* include/svpf.cuh:552-558 */
+typedef struct {
+    float log_lik_increment;
+    float vol_mean;
+    float vol_std;
+    float h_mean;
+    float mu_estimate;
+} SVPFResult;
+
+/* include/svpf.cuh:573 -- NULL on any failure (reference: malloc failure only). */
+SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t stream);
+/* include/svpf.cuh:578 */
+void svpf_destroy(SVPFState* state);
+/* include/svpf.cuh:587 -- particles drawn by the counter-based generator keyed by seed. */
+void svpf_initialize(SVPFState* state, const SVPFParams* params, unsigned long long seed);
+/* include/svpf.cuh:597 -- uses state->y_prev as y_{t-1}. */
+void svpf_step(SVPFState* state, float y_t, const SVPFParams* params, SVPFResult* result);
+/* include/svpf.cuh:602-603 -- rng_seed re-keys the generator for this tick. */
+void svpf_step_seeded(SVPFState* state, float y_t, const SVPFParams* params, unsigned long long rng_seed,
+                      SVPFResult* result);
+/* include/svpf.cuh:684-691 */
+void svpf_run_sequence(SVPFState* state, const float* h_observations, int T, const SVPFParams* params,
+                       float* h_loglik_out, float* h_vol_out);
+/* include/svpf.cuh:701-708 -- a device loop: one launch advances all T ticks. */
+void svpf_run_sequence_device(SVPFState* state, const float* d_observations, int T, const SVPFParams* params,
+                              float* d_loglik_out, float* d_vol_out);
+/* include/svpf.cuh:767-775 */
+void svpf_step_adaptive(SVPFState* state, float y_t, float y_prev, const SVPFParams* params, float* h_loglik_out,
+                        float* h_vol_out, float* h_mean_out);
+/* include/svpf.cuh:843-851 */
+void svpf_step_graph(SVPFState* state, float y_t, float y_prev, const SVPFParams* params, float* h_loglik_out,
+                     float* h_vol_out, float* h_mean_out);
+/* include/svpf.cuh:856 */
+bool svpf_graph_is_captured(SVPFState* state);
+/* include/svpf.cuh:869 */
+void svpf_graph_invalidate(SVPFState* state);
+/* include/svpf.cuh:795 */
+void svpf_get_particles(const SVPFState* state, float* h_out);
+/* include/svpf.cuh:800 */
+void svpf_get_stats(const SVPFState* state, float* h_mean, float* h_std);
+/* include/svpf.cuh:811 */
+float svpf_get_ess(const SVPFState* state);
+/* src/svpf_optimized_graph.cu:1490-1493 (defined there, never declared) */
+void svpf_get_ksd_stats(const SVPFState* state, float* ksd_out, int* steps_used_out);
+
+/* No reference counterpart: status of the last call on this state (SVPF_OK ...). */
+int svpf_last_error(const SVPFState* state);
+
+/* No reference counterpart: parity hooks for identical-input comparisons.
+ * svpf_set_particles uploads h, h_prev and the RMSProp second moment (any may
+ * be NULL: h_prev defaults to h, v to zero).  svpf_set_predrawn_noise points
+ * the step at device arrays of pre-drawn noise laid out [T][n]
+ * (z_pred, u_pred, u_rej, z_rej) and [T][xi_rows][n] (xi); tick k of the
+ * filter reads row k - t_base.  Passing z_pred = NULL returns to the
+ * counter-based generator.  svpf_set_scalar_state sets the device scalars
+ * the reference keeps on the device (mean of h after the last tick and the
+ * EMA bandwidth state). */
+int svpf_set_particles(SVPFState* state, const float* h, const float* h_prev, const float* v);
+int svpf_get_rmsprop_state(const SVPFState* state, float* v_out);
+int svpf_set_predrawn_noise(SVPFState* state, const float* d_z_pred, const float* d_u_pred, const float* d_xi,
+                            int xi_rows, const float* d_u_rej, const float* d_z_rej, int t_base);
+int svpf_set_scalar_state(SVPFState* state, float h_mean_prev, float bw_sq_ema, float return_ema,
+                          float return_var);
+/* No reference counterpart: per-tick diagnostics of the last tick
+ * (loglik, vol, h_mean, ksd, bandwidth, predictive loglik, steps, ell^2). */
+int svpf_get_last_outputs(const SVPFState* state, float out8[8]);
+
+/* include/svpf.cuh:626-632 */
+static inline void svpf_set_stein_sign_mode(SVPFState* state, int mode) {
+    state->stein_repulsive_sign = (mode == SVPF_STEIN_SIGN_PAPER) ? 1 : 0;
+}
+/* include/svpf.cuh:638-640 */
+static inline int svpf_get_stein_sign_mode(const SVPFState* state) { return state->stein_repulsive_sign; }
+/* include/svpf.cuh:661-667 */
+static inline void svpf_set_fan_mode(SVPFState* state, int enable) {
+    state->use_fan_mode = enable ? 1 : 0;
+    if (enable) state->stein_repulsive_sign = SVPF_STEIN_SIGN_PAPER;
+}
+/* include/svpf.cuh:673-675 */
+static inline int svpf_get_fan_mode(const SVPFState* state) { return state->use_fan_mode; }
+
+/* ------------------------------------------------------------------------ */
+/* Batched form (no reference counterpart: the reference runs one filter per
+ * SVPFState).  B independent filters with their own particle counts and
+ * parameters advance through ticks [t0, t1) in persistent launches, one per
+ * tile group.  All pointers are device pointers except group arrays.  The
+ * record layouts are those of newton-stein-variational-particle-filter_amd/
+ * csrc/svpf_types.h (svpf_fparams, svpf_fstate, svpf_fdesc).  order lists
+ * filter indices grouped by tile; group g covers order[group_start[g] ..
+ * group_start[g+1]) and runs with tile group_tile[g] and block size
+ * group_bs[g] on grid group_grid[g] workgroups.  queues holds one int per
+ * group. */
+int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, float* d_pool_h, float* d_pool_hp,
+                     float* d_pool_v, const int* d_order, const int* group_start, const int* group_tile,
+                     const int* group_bs, const int* group_grid, const int* group_nmax, int n_groups, int* d_queues,
+                     int t0, int t1, hipStream_t stream);
+/* Largest particle count a tile of the given block size holds (0: unknown tile). */
+int svpf_tile_capacity_query(int tile, int block_size);
+/* Dynamic LDS bytes a workgroup needs for filters of up to nmax particles. */
+size_t svpf_lds_bytes_query(int nmax);
+/* Library version string. */
+const char* svpf_version(void);
+
+/* include/svpf.cuh:1219-1256 */
+static inline float svpf_constrain_rho(float eta) { return tanhf(eta); }
+static inline float svpf_unconstrain_rho(float rho) {
+    rho = fminf(fmaxf(rho, -0.999f), 0.999f);
+    return 0.5f * logf((1.0f + rho) / (1.0f - rho));
+}
+static inline float svpf_constrain_sigma(float kappa) { return expf(kappa); }
+static inline float svpf_unconstrain_sigma(float sigma) { return logf(fmaxf(sigma, 1e-8f)); }
+static inline float svpf_constrain_nu(float kappa_nu) { return 2.0f + expf(kappa_nu); }
+static inline float svpf_unconstrain_nu(float nu) { return logf(fmaxf(nu - 2.0f, 1e-8f)); }
+static inline float svpf_drho_deta(float rho) { return 1.0f - rho * rho; }
+static inline float svpf_dsigma_dkappa(float sigma) { return sigma; }
+static inline float svpf_dnu_dkappa_nu(float nu) { return nu - 2.0f; }
+
+#ifdef __cplusplus
+}
+#endif
+#endif

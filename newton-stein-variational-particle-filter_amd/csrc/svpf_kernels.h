@@ -1,0 +1,37 @@
+// svpf_kernels.h -- host-side launch interface of the SVPF kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+
+#include "svpf_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Tile shapes of the single-workgroup engine: R rows per lane, CS lanes per row.
+enum {
+    SVPF_TILE_R1_C4 = 0,
+    SVPF_TILE_R1_C2 = 1,
+    SVPF_TILE_R1_C1 = 2,
+    SVPF_TILE_R2_C1 = 3,
+    SVPF_TILE_R4_C1 = 4,
+    SVPF_TILE_R8_C1 = 5,
+    SVPF_TILE_R2_C2 = 6,
+    SVPF_TILE_R4_C2 = 7,
+    SVPF_NUM_TILES = 8
+};
+
+size_t svpf_lds_bytes(int nmax);
+hipError_t svpf_kernels_init(void);
+hipError_t svpf_launch_init(float* h, float* hp, float* v, int n, float mu, float sd, uint64_t seed,
+                            hipStream_t stream);
+int svpf_tile_capacity(int tile, int bs);
+hipError_t svpf_launch_batched(int bs, int tile, int grid, const svpf_fparams* params, svpf_fstate* fstate,
+                               const svpf_fdesc* desc, float* pool_h, float* pool_hp, float* pool_v,
+                               const int* order, int nf, int* queue, int nmax, int t0, int t1, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,112 @@
+// svpf_types.h -- plain-old-data blocks shared by the HIP kernels and the host
+// layer.  No HIP or torch types: the same header is read by the C-ABI host code
+// and by the kernels.
+//
+// The per-filter parameter block carries the SVPFState feature flags and model
+// parameters that the reference's step reads (include/svpf.cuh:290-547,
+// src/svpf_optimized_graph.cu:915-1389), flattened so one block describes one
+// filter of a batch.
+#ifndef SVPF_TYPES_H
+#define SVPF_TYPES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+// Feature flags (svpf_fparams.flags).
+enum {
+    SVPF_F_EXACT_GRAD = 1u << 0,
+    SVPF_F_NEWTON = 1u << 1,
+    SVPF_F_FULL_NEWTON = 1u << 2,
+    SVPF_F_SVLD = 1u << 3,
+    SVPF_F_ANNEAL = 1u << 4,
+    SVPF_F_ADAPT_BETA = 1u << 5,
+    SVPF_F_LINEAR_BETA = 1u << 6,
+    SVPF_F_MIM = 1u << 7,
+    SVPF_F_ASYM_RHO = 1u << 8,
+    SVPF_F_LOCAL_PARAMS = 1u << 9,
+    SVPF_F_GUIDED = 1u << 10,
+    SVPF_F_GUIDE = 1u << 11,
+    SVPF_F_GUIDE_PRESERVING = 1u << 12,
+    SVPF_F_ADAPT_GUIDE = 1u << 13,
+    SVPF_F_REJUV = 1u << 14,
+    SVPF_F_FAN = 1u << 15,
+    SVPF_F_ADAPT_MU = 1u << 16,
+    SVPF_F_ADAPT_SIGMA = 1u << 17,
+    SVPF_F_KSD_EARLY_STOP = 1u << 18,
+    SVPF_F_PAPER_SIGN = 1u << 19,
+};
+
+// Bandwidth rules.
+enum { SVPF_BW_REFERENCE_EMA = 0, SVPF_BW_MEDIAN = 1, SVPF_BW_SILVERMAN = 2 };
+
+// Per-filter constant parameters (64 x 4 bytes).
+typedef struct svpf_fparams {
+    int32_t n;
+    uint32_t flags;
+    int32_t n_anneal;
+    int32_t stein_min;
+    int32_t stein_max;
+    int32_t bandwidth_mode;
+    int32_t pad0[2];
+    float mu, rho, sigma_z, gamma;
+    float nu, lik_offset, temperature, rms_rho;
+    float rms_eps, mim_p, mim_scale, rho_up;
+    float rho_down, delta_rho, delta_sigma, ga_base;
+    float ga_shock, ga_thr, gs_base, gs_max;
+    float gs_thr, rej_thr, rej_p, rej_blend;
+    float mu_q, mu_rscale, mu_min, mu_max;
+    float sb_thr, sb_max, ksd_thr, st_const;
+    float implied_offset, base_step;
+    float pad1[22];
+} svpf_fparams;
+
+// Per-filter scalar state carried between ticks (host fields of SVPFState and
+// the device scalars of SVPFOptimizedState).  32 x 4 bytes.
+typedef struct svpf_fstate {
+    int32_t timestep;
+    int32_t guide_initialized;
+    int32_t stein_steps_used;
+    int32_t status;            // non-zero: the kernel rejected this filter
+    float ksd_prev, vol_prev, h_mean_prev, guide_mean;
+    float guide_var, bw_sq_ema, return_ema, return_var;
+    float mu_state, mu_var, sigma_z_eff, y_prev;
+    float last_loglik, last_vol, last_h_mean, last_bw;
+    float last_ksd, last_pred_loglik, last_guide_strength, pad;
+    float pad2[8];
+} svpf_fstate;
+
+// Per-tick output record (8 floats): written to out[t * SVPF_OUTW + k].
+enum {
+    SVPF_O_LOGLIK = 0, SVPF_O_VOL = 1, SVPF_O_HMEAN = 2, SVPF_O_KSD = 3,
+    SVPF_O_BW = 4, SVPF_O_PRED_LL = 5, SVPF_O_STEPS = 6, SVPF_O_BWSQ = 7, SVPF_OUTW = 8
+};
+
+// Per-filter descriptor for one launch.
+typedef struct svpf_fdesc {
+    int64_t off;               // first particle of this filter in the particle pool
+    const float* y;            // observations, y[t] for t in [0, T)
+    float* out;                // per-tick records, out[t * SVPF_OUTW + k] (may be null)
+    const float* z_pred;       // pre-drawn predict normals [T][n], null: counter generator
+    const float* u_pred;       // pre-drawn MIM selector uniforms [T][n] (may be null)
+    const float* xi;           // pre-drawn Langevin normals [T][xi_rows][n]
+    const float* u_rej;        // pre-drawn rejuvenation uniforms [T][n] (may be null)
+    const float* z_rej;        // pre-drawn rejuvenation normals [T][n] (may be null)
+    float* log_w;              // final log weights [n] written at the end of a launch (may be null)
+    float* h_pred;             // particles after predict + guide of each tick [n] (may be null)
+    uint64_t seed;             // counter-generator key
+    int32_t xi_rows;
+    int32_t param_idx;         // index into the svpf_fparams array
+    uint32_t filter_id;        // counter-generator filter word
+    int32_t tile;              // tile-shape index chosen for this filter's n
+} svpf_fdesc;
+
+// Noise purposes: third word of the counter-generator counter.
+enum { SVPF_PURPOSE_PREDICT = 0, SVPF_PURPOSE_REJUV = 1, SVPF_PURPOSE_INIT = 255, SVPF_PURPOSE_SVLD0 = 16 };
+
+#ifdef __cplusplus
+}
+#endif
+#endif

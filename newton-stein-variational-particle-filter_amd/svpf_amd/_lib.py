@@ -1,0 +1,164 @@
+"""ctypes binding of libsvpf_amd.so (the C ABI declared in include/svpf.h).
+
+The library is the product path: every call here runs the HIP kernels.  There
+is no CPU fallback -- loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libsvpf_amd.so")
+
+SMOOTH_MAX_LAG = 8
+
+c_int, c_float, c_void_p = ctypes.c_int, ctypes.c_float, ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
+
+
+class SVPFParams(ctypes.Structure):
+    _fields_ = [("rho", c_float), ("sigma_z", c_float), ("mu", c_float), ("gamma", c_float)]
+
+
+class SVPFResult(ctypes.Structure):
+    _fields_ = [("log_lik_increment", c_float), ("vol_mean", c_float), ("vol_std", c_float),
+                ("h_mean", c_float), ("mu_estimate", c_float)]
+
+
+# Field order of SVPFState in include/svpf.h.
+_STATE_FIELDS = [
+    ("h", _FP), ("h_prev", _FP), ("h_pred", _FP), ("grad_log_p", _FP), ("log_weights", _FP), ("d_grad_v", _FP),
+    ("n_particles", c_int), ("n_stein_steps", c_int), ("nu", c_float), ("student_t_const", c_float),
+    ("student_t_implied_offset", c_float), ("timestep", c_int), ("y_prev", c_float), ("stream", c_void_p),
+    ("lik_offset", c_float), ("use_exact_gradient", c_int),
+    ("use_svld", c_int), ("use_annealing", c_int), ("use_adaptive_beta", c_int), ("n_anneal_steps", c_int),
+    ("temperature", c_float), ("rmsprop_rho", c_float), ("rmsprop_eps", c_float),
+    ("use_mim", c_int), ("mim_jump_prob", c_float), ("mim_jump_scale", c_float),
+    ("use_local_params", c_int), ("delta_rho", c_float), ("delta_sigma", c_float),
+    ("use_asymmetric_rho", c_int), ("rho_up", c_float), ("rho_down", c_float),
+    ("use_newton", c_int), ("use_full_newton", c_int),
+    ("use_guided", c_int), ("guided_alpha_base", c_float), ("guided_alpha_shock", c_float),
+    ("guided_innovation_threshold", c_float),
+    ("use_rejuvenation", c_int), ("rejuv_ksd_threshold", c_float), ("rejuv_prob", c_float), ("rejuv_blend", c_float),
+    ("use_guide", c_int), ("use_guide_preserving", c_int), ("guide_strength", c_float), ("guide_mean", c_float),
+    ("guide_var", c_float), ("guide_K", c_float), ("guide_initialized", c_int),
+    ("use_adaptive_guide", c_int), ("guide_strength_base", c_float), ("guide_strength_max", c_float),
+    ("guide_innovation_threshold", c_float), ("vol_prev", c_float),
+    ("use_adaptive_mu", c_int), ("mu_state", c_float), ("mu_var", c_float), ("mu_process_var", c_float),
+    ("mu_obs_var_scale", c_float), ("mu_min", c_float), ("mu_max", c_float),
+    ("use_adaptive_sigma", c_int), ("sigma_boost_threshold", c_float), ("sigma_boost_max", c_float),
+    ("sigma_z_effective", c_float),
+    ("stein_repulsive_sign", c_int), ("use_fan_mode", c_int),
+    ("stein_min_steps", c_int), ("stein_max_steps", c_int), ("ksd_improvement_threshold", c_float),
+    ("ksd_prev", c_float), ("stein_steps_used", c_int),
+    ("use_student_t_state", c_int), ("nu_state", c_float),
+    ("use_smoothing", c_int), ("smooth_lag", c_int), ("smooth_output_lag", c_int), ("smooth_head", c_int),
+    ("smooth_h_mean", c_float * SMOOTH_MAX_LAG), ("smooth_h_var", c_float * SMOOTH_MAX_LAG),
+    ("smooth_y", c_float * SMOOTH_MAX_LAG),
+    ("use_persistent_kernel", c_int), ("persistent_kernel_supported", c_int), ("use_heun", c_int),
+    ("use_two_factor", c_int),
+    ("bandwidth_mode", c_int), ("use_linear_beta", c_int), ("use_ksd_early_stop", c_int),
+    ("impl", c_void_p),
+]
+
+
+class SVPFState(ctypes.Structure):
+    _fields_ = _STATE_FIELDS
+
+
+# Status codes (include/svpf.h)
+SVPF_OK = 0
+SVPF_ERR_INVALID_ARGUMENT = 1
+SVPF_ERR_UNSUPPORTED = 2
+SVPF_ERR_HIP = 3
+SVPF_ERR_NOT_INITIALIZED = 4
+SVPF_ERR_OUT_OF_MEMORY = 5
+
+# Every symbol include/svpf.h declares (non-inline).
+EXPORTED = (
+    "svpf_create", "svpf_destroy", "svpf_initialize", "svpf_step", "svpf_step_seeded", "svpf_run_sequence",
+    "svpf_run_sequence_device", "svpf_step_adaptive", "svpf_step_graph", "svpf_graph_is_captured",
+    "svpf_graph_invalidate", "svpf_get_particles", "svpf_get_stats", "svpf_get_ess", "svpf_get_ksd_stats",
+    "svpf_last_error", "svpf_set_particles", "svpf_get_rmsprop_state", "svpf_set_predrawn_noise",
+    "svpf_set_scalar_state", "svpf_get_last_outputs", "svpf_batched_run", "svpf_tile_capacity_query",
+    "svpf_lds_bytes_query", "svpf_version",
+)
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load the library and declare signatures.  Raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libsvpf_amd.so not built ({p}); run __graft_entry__.build()")
+    L = ctypes.CDLL(p)
+    SP = ctypes.POINTER(SVPFState)
+    PP = ctypes.POINTER(SVPFParams)
+    L.svpf_create.restype = SP
+    L.svpf_create.argtypes = [c_int, c_int, c_float, c_void_p]
+    L.svpf_destroy.restype = None
+    L.svpf_destroy.argtypes = [SP]
+    L.svpf_initialize.restype = None
+    L.svpf_initialize.argtypes = [SP, PP, ctypes.c_ulonglong]
+    L.svpf_step.restype = None
+    L.svpf_step.argtypes = [SP, c_float, PP, ctypes.POINTER(SVPFResult)]
+    L.svpf_step_seeded.restype = None
+    L.svpf_step_seeded.argtypes = [SP, c_float, PP, ctypes.c_ulonglong, ctypes.POINTER(SVPFResult)]
+    L.svpf_run_sequence.restype = None
+    L.svpf_run_sequence.argtypes = [SP, c_void_p, c_int, PP, c_void_p, c_void_p]
+    L.svpf_run_sequence_device.restype = None
+    L.svpf_run_sequence_device.argtypes = [SP, c_void_p, c_int, PP, c_void_p, c_void_p]
+    for name in ("svpf_step_adaptive", "svpf_step_graph"):
+        f = getattr(L, name)
+        f.restype = None
+        f.argtypes = [SP, c_float, c_float, PP, _FP, _FP, _FP]
+    L.svpf_graph_is_captured.restype = ctypes.c_bool
+    L.svpf_graph_is_captured.argtypes = [SP]
+    L.svpf_graph_invalidate.restype = None
+    L.svpf_graph_invalidate.argtypes = [SP]
+    L.svpf_get_particles.restype = None
+    L.svpf_get_particles.argtypes = [SP, c_void_p]
+    L.svpf_get_stats.restype = None
+    L.svpf_get_stats.argtypes = [SP, _FP, _FP]
+    L.svpf_get_ess.restype = c_float
+    L.svpf_get_ess.argtypes = [SP]
+    L.svpf_get_ksd_stats.restype = None
+    L.svpf_get_ksd_stats.argtypes = [SP, _FP, ctypes.POINTER(c_int)]
+    L.svpf_last_error.restype = c_int
+    L.svpf_last_error.argtypes = [SP]
+    L.svpf_set_particles.restype = c_int
+    L.svpf_set_particles.argtypes = [SP, c_void_p, c_void_p, c_void_p]
+    L.svpf_get_rmsprop_state.restype = c_int
+    L.svpf_get_rmsprop_state.argtypes = [SP, c_void_p]
+    L.svpf_set_predrawn_noise.restype = c_int
+    L.svpf_set_predrawn_noise.argtypes = [SP, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int]
+    L.svpf_set_scalar_state.restype = c_int
+    L.svpf_set_scalar_state.argtypes = [SP, c_float, c_float, c_float, c_float]
+    L.svpf_get_last_outputs.restype = c_int
+    L.svpf_get_last_outputs.argtypes = [SP, c_void_p]
+    L.svpf_batched_run.restype = c_int
+    L.svpf_batched_run.argtypes = [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                                   c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int,
+                                   c_void_p]
+    L.svpf_tile_capacity_query.restype = c_int
+    L.svpf_tile_capacity_query.argtypes = [c_int, c_int]
+    L.svpf_lds_bytes_query.restype = ctypes.c_size_t
+    L.svpf_lds_bytes_query.argtypes = [c_int]
+    L.svpf_version.restype = ctypes.c_char_p
+    L.svpf_version.argtypes = []
+    if path is None:
+        _lib = L
+    return L
+
+
+def check(state_ptr, what: str):
+    rc = load().svpf_last_error(state_ptr)
+    if rc != SVPF_OK:
+        names = {1: "invalid argument", 2: "unsupported feature", 3: "HIP runtime error",
+                 4: "filter not initialized", 5: "out of memory"}
+        raise RuntimeError(f"{what} failed: {names.get(rc, rc)}")

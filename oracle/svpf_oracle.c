@@ -1,0 +1,606 @@
+/*
+ * svpf_oracle.c -- scalar FP64 restatement of the SVPF per-tick step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Loaded by
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg; never by
+ * the product.
+ *
+ * Follows (paths relative to the reference repository root):
+ *   src/svpf_optimized_graph.cu:110-302   create defaults, Student-t constants
+ *   src/svpf_optimized_graph.cu:915-1389  svpf_step_graph orchestration
+ *   src/svpf_opt_kernels.cu               predict / guide / gradient / transport / KSD / outputs
+ *   include/svpf_kernels.cuh:509-541      host EKF guide
+ *
+ * Semantics fixed here where the reference is ambiguous:
+ *   - Stein transport is a Jacobi update: every phi_i uses the positions of
+ *     the iteration's start (the reference reads them from a shared-memory
+ *     snapshot, src/svpf_opt_kernels.cu:548-553).
+ *   - bandwidth_mode MEDIAN: ell^2 = lower median of the N(N-1)/2 pairwise
+ *     squared distances (index floor((M-1)/2) of the ascending order).
+ *   - use_linear_beta: beta_k = 0.3 + 0.7 k/(K-1) over K annealing stages.
+ *   - use_ksd_early_stop: the KSD is evaluated every iteration and the loop
+ *     stops once the relative improvement drops below
+ *     ksd_improvement_threshold (the field exists at include/svpf.cuh:509 but
+ *     the reference never reads it).
+ */
+#include "svpf_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+static double clamp_d(double x, double lo, double hi) { return x < lo ? lo : (x > hi ? hi : x); }
+/* src/svpf_opt_kernels.cu:20-26 */
+static double clamp_logvol(double h) { return clamp_d(h, -15.0, 5.0); }
+static double safe_exp(double x) { return exp(x < 20.0 ? x : 20.0); }
+
+/* ------------------------------------------------------------------------ */
+/* Counter-based generator (Philox4x32-10, Salmon, Moraes, Dror, Shaw 2011). */
+/* ------------------------------------------------------------------------ */
+void svpfo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Uniform on (0,1]: 24 random bits, exactly representable in float32. */
+double svpfo_u01_from_bits(uint32_t x) { return (double)((x >> 8) + 1u) * (1.0 / 16777216.0); }
+
+/* Box-Muller (cosine branch) from two 32-bit words. */
+double svpfo_normal_from_bits(uint32_t a, uint32_t b) {
+    double u1 = svpfo_u01_from_bits(a);
+    double u2 = (double)(b >> 8) * (1.0 / 16777216.0);
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * M_PI * u2);
+}
+
+/* Noise purposes (third counter word). */
+enum { PURPOSE_PREDICT = 0, PURPOSE_REJUV = 1, PURPOSE_INIT = 255, PURPOSE_SVLD0 = 16 };
+
+static void philox_draw(uint64_t seed, uint32_t filter, uint32_t tick, uint32_t purpose, uint32_t i,
+                        uint32_t out[4]) {
+    uint32_t ctr[4] = {i, tick, purpose, filter};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    svpfo_philox4x32_10(ctr, key, out);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Closed forms                                                              */
+/* ------------------------------------------------------------------------ */
+
+/* src/svpf_optimized_graph.cu:123-125 */
+double svpfo_student_t_const(double nu) {
+    return lgamma((nu + 1.0) / 2.0) - lgamma(nu / 2.0) - 0.5 * log(M_PI * nu);
+}
+
+/* src/svpf_optimized_graph.cu:127-138: -(log nu + psi(1/2) - psi_hat(nu/2)),
+ * psi_hat the 2-term asymptotic (or the pole form below 1). */
+double svpfo_implied_offset(double nu) {
+    const double psi_half = -1.9635100260214235;
+    double nh = nu / 2.0, psi_nh;
+    if (nh >= 1.0) psi_nh = log(nh) - 1.0 / (2.0 * nh) - 1.0 / (12.0 * nh * nh);
+    else psi_nh = -0.5772156649 - 1.0 / nh;
+    return -(log(nu) + psi_half - psi_nh);
+}
+
+/* src/svpf_opt_kernels.cu:474-488: Student-t log weight with the 1e-8 guard. */
+double svpfo_log_w(double h, double y, double nu, double st_const) {
+    double vol = safe_exp(h);
+    double A = (y * y) / (vol + 1e-8) / nu;
+    double a = A > -0.999 ? A : -0.999;
+    return st_const - 0.5 * h - (nu + 1.0) * 0.5 * log1p(a);
+}
+
+static int cmp_double(const void* a, const void* b) {
+    double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+static int cmp_float(const void* a, const void* b) {
+    float x = *(const float*)a, y = *(const float*)b;
+    return (x > y) - (x < y);
+}
+
+/* Lower median of the pairwise squared distances, by full sort. */
+double svpfo_median_r2(const double* h, int n) {
+    if (n < 2) return 0.0;
+    size_t m = (size_t)n * (size_t)(n - 1) / 2, k = 0;
+    double* r = (double*)malloc(m * sizeof(double));
+    if (!r) return NAN;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) { double d = h[i] - h[j]; r[k++] = d * d; }
+    qsort(r, m, sizeof(double), cmp_double);
+    double med = r[(m - 1) / 2];
+    free(r);
+    return med;
+}
+
+/* Same selection in float32 arithmetic (d = fl(h_i - h_j), r = fl(d*d)); the
+ * bit pattern of the result is what the device selection must reproduce. */
+float svpfo_median_r2_f32(const float* h, int n) {
+    if (n < 2) return 0.0f;
+    size_t m = (size_t)n * (size_t)(n - 1) / 2, k = 0;
+    float* r = (float*)malloc(m * sizeof(float));
+    if (!r) return NAN;
+    for (int i = 0; i < n; ++i)
+        for (int j = i + 1; j < n; ++j) { volatile float d = h[i] - h[j]; volatile float q = d * d; r[k++] = q; }
+    qsort(r, m, sizeof(float), cmp_float);
+    float med = r[(m - 1) / 2];
+    free(r);
+    return med;
+}
+
+/* src/svpf_optimized_graph.cu:1066-1092 */
+double svpfo_adaptive_beta(double ksd_prev, int ai, int n_anneal) {
+    if (ai == n_anneal - 1) return 1.0;
+    const double ksd_high = 0.50, ksd_low = 0.05, beta_min = 0.30, beta_mid = 0.80;
+    double beta;
+    if (ksd_prev > ksd_high) beta = beta_min;
+    else if (ksd_prev < ksd_low) beta = beta_mid;
+    else {
+        double t = (ksd_high - ksd_prev) / (ksd_high - ksd_low);
+        beta = beta_min + t * (beta_mid - beta_min);
+    }
+    double progress = (double)(ai + 1) / (double)n_anneal;
+    beta *= (0.5 + 0.5 * progress);
+    return clamp_d(beta, 0.1, 1.0);
+}
+
+/* src/svpf_optimized_graph.cu:1097-1112 */
+int svpfo_stein_budget(const svpfo_cfg* c, const svpfo_state* s) {
+    if (s->timestep < 10) return c->stein_max_steps;
+    double x = (s->ksd_prev - 0.05) / (0.50 - 0.05);
+    x = clamp_d(x, 0.0, 1.0);
+    /* float32 in the reference: (int)(x * (max-min)) truncates toward zero */
+    return c->stein_min_steps + (int)((float)x * (float)(c->stein_max_steps - c->stein_min_steps));
+}
+
+/* include/svpf_kernels.cuh:509-541 */
+void svpfo_ekf_update(const svpfo_cfg* c, svpfo_state* s, double y_t, double mu, double rho, double sigma_z) {
+    (void)c;
+    if (!s->guide_initialized) {
+        s->guide_mean = mu;
+        s->guide_var = sigma_z * sigma_z / (1.0 - rho * rho);
+        s->guide_initialized = 1;
+    }
+    double m_pred = mu + rho * (s->guide_mean - mu);
+    double P_pred = rho * rho * s->guide_var + sigma_z * sigma_z;
+    double log_y2 = log(y_t * y_t + 1e-8);
+    double obs_offset = -svpfo_implied_offset(c->nu);
+    double R = 4.93 + 2.0;
+    double S = P_pred + R;
+    double K = P_pred / (S + 1e-8);
+    double innov = log_y2 - (m_pred + obs_offset);
+    s->guide_mean = m_pred + K * innov;
+    s->guide_var = (1.0 - K) * P_pred;
+}
+
+/* ------------------------------------------------------------------------ */
+/* O(N^2) pieces                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* src/svpf_opt_kernels.cu:354-521 (Gaussian state prior branch). */
+void svpfo_gradient(const svpfo_cfg* c, const double* h, const double* h_prev, double y_t, double y_prev,
+                    double mu, double sigma_z, double beta,
+                    double* grad, double* log_w, double* precond, double* curv) {
+    int n = c->n;
+    double rho = c->rho, gamma = c->gamma, nu = c->nu;
+    double st_const = svpfo_student_t_const(nu);
+    double* m = (double*)malloc((size_t)n * sizeof(double));
+    for (int k = 0; k < n; ++k) {
+        double hp = h_prev[k];
+        double lev = gamma * y_prev / (exp(hp * 0.5) + 1e-8);
+        m[k] = mu + rho * (hp - mu) + lev;
+    }
+    double s2 = sigma_z * sigma_z;
+    double inv_2s2 = 0.5 / s2, inv_s2 = 1.0 / s2;
+    for (int j = 0; j < n; ++j) {
+        double hj = h[j];
+        double lmax = -1e10;
+        for (int i = 0; i < n; ++i) { double d = hj - m[i]; double l = -d * d * inv_2s2; if (l > lmax) lmax = l; }
+        double sum_r = 0.0, wg = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double d = hj - m[i];
+            double r = exp(-d * d * inv_2s2 - lmax);
+            sum_r += r;
+            wg -= r * d * inv_s2;
+        }
+        double g_prior = wg / (sum_r + 1e-8);
+        double h_prior = -inv_s2;
+
+        double vol = safe_exp(hj);
+        double ysq = y_t * y_t;
+        double A = ysq / (vol + 1e-8) / nu;
+        double opa = 1.0 + A;
+        if (c->use_fan_mode) log_w[j] = 0.0;
+        else log_w[j] = svpfo_log_w(hj, y_t, nu, st_const);
+        double g_lik;
+        if (c->use_exact_gradient) g_lik = -0.5 + 0.5 * (nu + 1.0) * A / opa - c->lik_offset;
+        else g_lik = (log(ysq + 1e-10) - hj + c->lik_offset) / 1.4;
+        double eb = c->use_fan_mode ? 1.0 : beta;
+        double g = clamp_d(g_prior + eb * g_lik, -10.0, 10.0);
+        grad[j] = g;
+        if (c->use_newton) {
+            double h_lik = -0.5 * (nu + 1.0) * A / (opa * opa);
+            double cv = clamp_d(-(h_lik + h_prior), 0.1, 100.0);
+            curv[j] = cv;
+            precond[j] = 0.7 * g / cv;
+        } else {
+            curv[j] = 1.0;
+            precond[j] = g;
+        }
+    }
+    free(m);
+}
+
+/* src/svpf_opt_kernels.cu:527-599 (plain), :742-906 (local Newton),
+ * :912-1103 (full Newton).  s is the score the transport uses: the raw
+ * gradient for plain and full Newton, the preconditioned one for local Newton.
+ * curv is used by local Newton (repulsion / curv_j) and full Newton (H-bar). */
+void svpfo_stein_phi(const svpfo_cfg* c, const double* h, const double* s, const double* curv,
+                     double inv_bw_sq, double* phi, double* ksd_partial) {
+    int n = c->n;
+    double sgn = (c->stein_sign == 1) ? 1.0 : -1.0;
+    int newton = c->use_newton && !c->use_full_newton;
+    int full = c->use_newton && c->use_full_newton;
+    for (int i = 0; i < n; ++i) {
+        double hi = h[i], si = s[i];
+        double ks = 0.0, gks = 0.0, ksd = 0.0, Hw = 0.0, Kn = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double d = hi - h[j];
+            double dist = d * d * inv_bw_sq;
+            double K = 1.0 / (1.0 + dist);
+            double K2 = K * K;
+            ks += K * s[j];
+            double rep = sgn * 2.0 * d * inv_bw_sq * K2;
+            if (newton) rep /= curv[j];
+            gks += rep;
+            if (full) {
+                Hw += curv[j] * K + 2.0 * inv_bw_sq * K2 * fabs(3.0 * dist - 1.0);
+                Kn += K;
+            }
+            double gx = -2.0 * d * inv_bw_sq * K2;
+            double gy = -gx;
+            double hxy = 2.0 * inv_bw_sq * K2 * (4.0 * dist * K - 1.0);
+            ksd += K * si * s[j] + si * gy + s[j] * gx + hxy;
+        }
+        double p = (ks + gks) / (double)n;
+        if (full) {
+            double Hb = clamp_d(Hw / (Kn > 1e-6 ? Kn : 1e-6), 0.1, 100.0);
+            p = p / Hb * 0.7;
+        }
+        phi[i] = p;
+        if (ksd_partial) ksd_partial[i] = ksd;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Bandwidth                                                                 */
+/* ------------------------------------------------------------------------ */
+
+/* Returns inv_bw_sq; writes bw and ell^2.  Reference EMA rule:
+ * src/svpf_opt_kernels.cu:1159-1235.  Silverman: src/svpf_fully_fused_v2.cu:397-424. */
+static double svpfo_bandwidth(const svpfo_cfg* c, svpfo_state* st, const double* h, double y_t,
+                              double* bw_out, double* raw_out) {
+    int n = c->n;
+    double bw, raw;
+    if (c->bandwidth_mode == SVPFO_BW_MEDIAN) {
+        double med = svpfo_median_r2(h, n);
+        bw = clamp_d(sqrt(med), 0.01, 2.0);
+        raw = med;
+    } else if (c->bandwidth_mode == SVPFO_BW_SILVERMAN) {
+        double sum = 0.0, sq = 0.0;
+        for (int i = 0; i < n; ++i) { sum += h[i]; sq += h[i] * h[i]; }
+        double mean = sum / n, var = sq / n - mean * mean;
+        double sd = sqrt(var > 1e-6 ? var : 1e-6);
+        bw = clamp_d(1.06 * sd * pow((double)n, -0.2), 0.05, 2.0);
+        raw = bw * bw;
+    } else {
+        double sum = 0.0, sq = 0.0, mn = 1e10, mx = -1e10;
+        for (int i = 0; i < n; ++i) {
+            double v = h[i];
+            sum += v; sq += v * v;
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+        }
+        double mean = sum / n, var = sq / n - mean * mean, spread = mx - mn;
+        double bw_sq_new = 2.0 * var / log((double)n + 1.0);
+        if (bw_sq_new < 1e-6) bw_sq_new = 1e-6;
+        double prev = st->bw_sq_ema;
+        double bw_sq = prev > 0.0 ? 0.3 * bw_sq_new + 0.7 * prev : bw_sq_new;
+        double ar = fabs(y_t);
+        double re = st->return_ema, rv = st->return_var;
+        re = re > 0.0 ? 0.05 * ar + 0.95 * re : ar;
+        rv = rv > 0.0 ? 0.05 * ar * ar + 0.95 * rv : ar * ar;
+        st->return_ema = re;
+        st->return_var = rv;
+        double vol_ratio = ar / (re > 1e-8 ? re : 1e-8);
+        double spread_factor = spread * 0.5 < 2.0 ? spread * 0.5 : 2.0;
+        double comb = vol_ratio > spread_factor ? vol_ratio : spread_factor;
+        double sc = 1.0 - 0.25 * (comb - 1.0 < 2.0 ? comb - 1.0 : 2.0);
+        sc = clamp_d(sc, 0.5, 1.0);
+        bw_sq *= sc;
+        st->bw_sq_ema = bw_sq;
+        bw = clamp_d(sqrt(bw_sq), 0.01, 2.0);
+        raw = bw_sq;
+    }
+    *bw_out = bw;
+    *raw_out = raw;
+    return 1.0 / (bw * bw);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tick                                                                      */
+/* ------------------------------------------------------------------------ */
+
+static double noise_normal(const svpfo_noise* nz, const float* arr, int i, uint32_t tick, uint32_t purpose) {
+    if (arr) return (double)arr[i];
+    uint32_t o[4];
+    philox_draw(nz->seed, nz->filter_id, tick, purpose, (uint32_t)i, o);
+    return svpfo_normal_from_bits(o[0], o[1]);
+}
+static double noise_uniform(const svpfo_noise* nz, const float* arr, int i, uint32_t tick, uint32_t purpose) {
+    if (arr) return (double)arr[i];
+    uint32_t o[4];
+    philox_draw(nz->seed, nz->filter_id, tick, purpose, (uint32_t)i, o);
+    return svpfo_u01_from_bits(o[2]);
+}
+
+static double lse_mean(const double* lw, int n) {
+    double mx = -1e10;
+    for (int i = 0; i < n; ++i) if (lw[i] > mx) mx = lw[i];
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += exp(lw[i] - mx);
+    double m = s / n;
+    return mx + log(m > 1e-10 ? m : 1e-10);
+}
+
+int svpfo_tick(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, double* v,
+               double y_t, double y_prev, const svpfo_noise* nz, svpfo_out* out) {
+    int n = c->n;
+    uint32_t tick = (uint32_t)st->timestep;
+    int rng = (nz->z_pred == NULL);
+    double* grad = (double*)calloc((size_t)n, sizeof(double));
+    double* log_w = (double*)calloc((size_t)n, sizeof(double));
+    double* prec = (double*)calloc((size_t)n, sizeof(double));
+    double* curv = (double*)calloc((size_t)n, sizeof(double));
+    double* phi = (double*)calloc((size_t)n, sizeof(double));
+    double* kp = (double*)calloc((size_t)n, sizeof(double));
+    double* hn = (double*)calloc((size_t)n, sizeof(double));
+    if (!grad || !log_w || !prec || !curv || !phi || !kp || !hn) return -1;
+
+    /* 1. effective mu / sigma (src/svpf_optimized_graph.cu:932-948) */
+    double mu = c->use_adaptive_mu ? st->mu_state : c->mu;
+    double sigma = c->sigma_z;
+    if (c->use_adaptive_sigma && st->timestep > 0) {
+        double vol_est = st->vol_prev > 1e-4 ? st->vol_prev : 1e-4;
+        double rz = fabs(y_t) / vol_est;
+        double boost = 1.0;
+        if (rz > c->sigma_boost_threshold) {
+            double sev = (rz - c->sigma_boost_threshold) / 3.0;
+            if (sev > 1.0) sev = 1.0;
+            boost = 1.0 + (c->sigma_boost_max - 1.0) * sev;
+        }
+        sigma = c->sigma_z * boost;
+        st->sigma_z_effective = sigma;
+    }
+    double rho_up = c->use_asymmetric_rho ? c->rho_up : c->rho;
+    double rho_down = c->use_asymmetric_rho ? c->rho_down : c->rho;
+    double d_rho = c->use_local_params ? c->delta_rho : 0.0;
+    double d_sig = c->use_local_params ? c->delta_sigma : 0.0;
+    double implied = svpfo_implied_offset(c->nu);
+
+    /* 2. predict (src/svpf_opt_kernels.cu:125-276) */
+    for (int i = 0; i < n; ++i) {
+        double hi = h[i], hpi = h_prev[i];
+        double z = noise_normal(nz, nz->z_pred, i, tick, PURPOSE_PREDICT);
+        double lev = c->gamma * y_prev / (safe_exp(hi / 2.0) + 1e-8);
+        double hnew;
+        if (c->use_guided || c->use_mim) {
+            double u = rng ? noise_uniform(nz, NULL, i, tick, PURPOSE_PREDICT)
+                           : (nz->u_pred ? (double)nz->u_pred[i] : 1.0);
+            double dev = hi - st->h_mean_prev;
+            double rho_adj = d_rho * tanh(dev);
+            double sig_scale = 1.0 + d_sig * fabs(dev);
+            double scale = (u < c->mim_jump_prob) ? c->mim_jump_scale : 1.0;
+            double base_rho = (hi > hpi) ? rho_up : rho_down;
+            double r = clamp_d(base_rho + rho_adj, 0.0, 0.999);
+            double sl = sigma * sig_scale;
+            double mean_prior = mu + r * (hi - mu) + lev;
+            if (c->use_guided) {
+                double log_y2 = log(y_t * y_t + 1e-10);
+                double mi = log_y2 + implied;
+                if (mi < -5.0) mi = -5.0;
+                double zs = (mi - mean_prior) / 2.5;
+                double act = zs > c->guided_innovation_threshold ? tanh(zs - c->guided_innovation_threshold) : 0.0;
+                double a = c->guided_alpha_base + (c->guided_alpha_shock - c->guided_alpha_base) * act;
+                double mp = (1.0 - a) * mean_prior + a * mi;
+                hnew = clamp_logvol(mp + sl * scale * z);
+            } else {
+                hnew = clamp_logvol(mean_prior + sl * scale * z);
+            }
+        } else {
+            hnew = clamp_logvol(mu + c->rho * (hi - mu) + sigma * z + lev);
+        }
+        h_prev[i] = hi;
+        h[i] = hnew;
+    }
+
+    /* predictive loglik before any transport (loss of the differentiable form) */
+    {
+        double stc = svpfo_student_t_const(c->nu);
+        for (int i = 0; i < n; ++i) log_w[i] = svpfo_log_w(h[i], y_t, c->nu, stc);
+        out->pred_loglik = lse_mean(log_w, n);
+    }
+
+    /* 3. guide (src/svpf_optimized_graph.cu:1006-1039) */
+    if (c->use_guide) {
+        double gs = c->guide_strength_base;
+        if (c->use_adaptive_guide && st->timestep > 0) {
+            double vol_est = st->vol_prev > 1e-4 ? st->vol_prev : 1e-4;
+            double rz = fabs(y_t) / vol_est;
+            double ih = log(y_t * y_t + 1e-8) + implied;
+            double he = log(vol_est * vol_est + 1e-8);
+            if (ih - he > 0.0 && rz > c->guide_innovation_threshold) {
+                double sev = (rz - c->guide_innovation_threshold) / 3.0;
+                if (sev > 1.0) sev = 1.0;
+                gs = c->guide_strength_base + (c->guide_strength_max - c->guide_strength_base) * sev;
+            }
+        }
+        svpfo_ekf_update(c, st, y_t, mu, c->rho, c->sigma_z);
+        for (int i = 0; i < n; ++i) {
+            if (c->use_guide_preserving) {
+                double cm = st->h_mean_prev;
+                double nm = (1.0 - gs) * cm + gs * st->guide_mean;
+                h[i] = clamp_logvol(nm + (h[i] - cm));
+            } else {
+                h[i] = clamp_logvol(h[i] + gs * (st->guide_mean - h[i]));
+            }
+        }
+    }
+
+    /* 4. bandwidth */
+    double bw, bw_raw;
+    double inv_bw_sq = svpfo_bandwidth(c, st, h, y_t, &bw, &bw_raw);
+
+    /* 5. annealed Stein loop (src/svpf_optimized_graph.cu:1057-1317) */
+    int n_anneal = c->use_annealing ? c->n_anneal_steps : 1;
+    if (n_anneal < 1) n_anneal = 1;
+    double base_step = 0.1 * (c->use_guide ? 0.5 : 1.0);
+    int budget = svpfo_stein_budget(c, st);
+    int total = 0, stop = 0;
+    double ksd = 0.0, ksd_last_iter = -1.0;
+    int use_newton = c->use_newton, full = c->use_newton && c->use_full_newton;
+    for (int ai = 0; ai < n_anneal && !stop; ++ai) {
+        double beta;
+        if (!c->use_annealing) beta = 1.0;
+        else if (c->use_linear_beta) beta = n_anneal > 1 ? 0.3 + 0.7 * (double)ai / (double)(n_anneal - 1) : 1.0;
+        else if (c->use_adaptive_beta) beta = svpfo_adaptive_beta(st->ksd_prev, ai, n_anneal);
+        else { static const double sch[3] = {0.3, 0.65, 1.0}; beta = sch[ai % 3]; }
+        double bf = sqrt(beta);
+        double temp = c->use_svld ? c->temperature : 0.0;
+        int si = budget / n_anneal;
+        if (ai == n_anneal - 1) si = budget - si * (n_anneal - 1);
+        for (int s = 0; s < si; ++s) {
+            int last = (ai == n_anneal - 1) && (s == si - 1);
+            int row = total;
+            total++;
+            svpfo_gradient(c, h, h_prev, y_t, y_prev, mu, sigma, beta, grad, log_w, prec, curv);
+            const double* sc = (use_newton && !full) ? prec : grad;
+            int want_ksd = last || c->use_ksd_early_stop;
+            svpfo_stein_phi(c, h, sc, curv, inv_bw_sq, phi, want_ksd ? kp : NULL);
+            double eff = base_step * bf;
+            for (int i = 0; i < n; ++i) {
+                double p = phi[i];
+                double vn = c->rmsprop_rho * v[i] + (1.0 - c->rmsprop_rho) * p * p;
+                v[i] = vn;
+                double drift = eff * p / sqrt(vn + c->rmsprop_eps);
+                double diff = 0.0;
+                if (temp > 1e-6) {
+                    double xi;
+                    if (rng) xi = noise_normal(nz, NULL, i, tick, PURPOSE_SVLD0 + (uint32_t)row);
+                    else xi = (row < nz->xi_rows) ? (double)nz->xi[(size_t)row * n + i] : 0.0;
+                    diff = sqrt(2.0 * eff * temp) * xi;
+                }
+                hn[i] = clamp_logvol(h[i] + drift + diff);
+            }
+            memcpy(h, hn, (size_t)n * sizeof(double));
+            if (want_ksd) {
+                double sum = 0.0;
+                for (int i = 0; i < n; ++i) sum += kp[i];
+                double k2 = sum / ((double)n * (double)n);
+                ksd = sqrt(k2 > 0.0 ? k2 : 0.0);
+                if (c->use_ksd_early_stop && ksd_last_iter > 0.0 && total >= c->stein_min_steps) {
+                    double rel = (ksd_last_iter - ksd) / ksd_last_iter;
+                    if (rel < c->ksd_improvement_threshold) { stop = 1; break; }
+                }
+                ksd_last_iter = ksd;
+            }
+        }
+    }
+    st->stein_steps_used = total;
+
+    /* 6. rejuvenation (src/svpf_optimized_graph.cu:1328-1341, src/svpf_opt_kernels.cu:1248-1275) */
+    if (c->use_rejuvenation && c->use_guide && st->timestep > 10 && st->ksd_prev > c->rejuv_ksd_threshold) {
+        double gsd = sqrt(st->guide_var > 1e-6 ? st->guide_var : 1e-6);
+        for (int i = 0; i < n; ++i) {
+            double u = noise_uniform(nz, nz->u_rej, i, tick, PURPOSE_REJUV);
+            if (u < c->rejuv_prob) {
+                double z = noise_normal(nz, nz->z_rej, i, tick, PURPOSE_REJUV);
+                h[i] = clamp_logvol((1.0 - c->rejuv_blend) * h[i] + c->rejuv_blend * (st->guide_mean + gsd * z));
+            }
+        }
+    }
+
+    /* 7. outputs (src/svpf_opt_kernels.cu:1109-1153) */
+    double sv = 0.0, sh = 0.0;
+    for (int i = 0; i < n; ++i) { sv += safe_exp(h[i] * 0.5); sh += h[i]; }
+    out->loglik = lse_mean(log_w, n);
+    out->vol = sv / n;
+    out->h_mean = sh / n;
+    out->ksd = ksd;
+    out->bandwidth = bw;
+    out->bw_sq = bw_raw;
+    out->steps = total;
+
+    /* 8. host tail (src/svpf_optimized_graph.cu:1383-1388) */
+    st->h_mean_prev = out->h_mean;
+    st->vol_prev = out->vol;
+    st->ksd_prev = ksd;
+    if (c->use_adaptive_mu && st->timestep > 10) {
+        double Pp = st->mu_var + c->mu_process_var;
+        double R = c->mu_obs_var_scale * bw * bw;
+        double K = Pp / (Pp + R + 1e-8);
+        double mn = st->mu_state + K * (out->h_mean - st->mu_state);
+        st->mu_state = clamp_d(mn, c->mu_min, c->mu_max);
+        st->mu_var = (1.0 - K) * Pp;
+    }
+    st->timestep++;
+
+    free(grad); free(log_w); free(prec); free(curv); free(phi); free(kp); free(hn);
+    return 0;
+}
+
+int svpfo_run(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, double* v,
+              int T, const float* y, float y_prev0,
+              const float* z_pred, const float* u_pred, const float* xi, int xi_rows,
+              const float* u_rej, const float* z_rej,
+              double* loglik, double* pred_loglik, double* vol, double* h_mean,
+              double* ksd, double* bw, int* steps) {
+    int n = c->n;
+    double yp = y_prev0;
+    for (int t = 0; t < T; ++t) {
+        svpfo_noise nz;
+        memset(&nz, 0, sizeof(nz));
+        nz.z_pred = z_pred ? z_pred + (size_t)t * n : NULL;
+        nz.u_pred = u_pred ? u_pred + (size_t)t * n : NULL;
+        nz.xi = xi ? xi + (size_t)t * xi_rows * n : NULL;
+        nz.xi_rows = xi_rows;
+        nz.u_rej = u_rej ? u_rej + (size_t)t * n : NULL;
+        nz.z_rej = z_rej ? z_rej + (size_t)t * n : NULL;
+        svpfo_out o;
+        int rc = svpfo_tick(c, st, h, h_prev, v, (double)y[t], yp, &nz, &o);
+        if (rc) return rc;
+        if (loglik) loglik[t] = o.loglik;
+        if (pred_loglik) pred_loglik[t] = o.pred_loglik;
+        if (vol) vol[t] = o.vol;
+        if (h_mean) h_mean[t] = o.h_mean;
+        if (ksd) ksd[t] = o.ksd;
+        if (bw) bw[t] = o.bandwidth;
+        if (steps) steps[t] = o.steps;
+        yp = (double)y[t];
+    }
+    return 0;
+}

@@ -1,0 +1,156 @@
+/*
+ * svpf_oracle.h -- scalar FP64 restatement of the SVPF per-tick step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
+ * path is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  Nothing in the product path links
+ * or calls it.
+ *
+ * Every function restates the reference algorithm in double precision and
+ * cites the reference file:line it follows (paths relative to the reference
+ * repository root).
+ */
+#ifndef SVPF_ORACLE_H
+#define SVPF_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Bandwidth rules (the reference has three; see svpfo_bandwidth). */
+enum { SVPFO_BW_REFERENCE_EMA = 0, SVPFO_BW_MEDIAN = 1, SVPFO_BW_SILVERMAN = 2 };
+
+/* Per-filter configuration: the SVPFState feature flags and parameters that the
+ * step reads (include/svpf.cuh:290-547) plus the three extensions the
+ * benchmark configurations need (bandwidth_mode, use_linear_beta,
+ * use_ksd_early_stop). */
+typedef struct svpfo_cfg {
+    int n;
+    int bandwidth_mode;
+    int use_exact_gradient;
+    int use_newton;
+    int use_full_newton;
+    int stein_sign;            /* 0 legacy (attract), 1 paper (repel) */
+    int use_svld;
+    int use_annealing;
+    int n_anneal_steps;
+    int use_adaptive_beta;
+    int use_linear_beta;       /* beta_k = 0.3 + 0.7 k/(K-1) */
+    int stein_min_steps;
+    int stein_max_steps;
+    int use_mim;
+    int use_asymmetric_rho;
+    int use_local_params;
+    int use_guided;
+    int use_guide;
+    int use_guide_preserving;
+    int use_adaptive_guide;
+    int use_rejuvenation;
+    int use_fan_mode;
+    int use_adaptive_mu;
+    int use_adaptive_sigma;
+    int use_ksd_early_stop;
+    int pad_;
+    double mu, rho, sigma_z, gamma;
+    double nu, lik_offset;
+    double temperature, rmsprop_rho, rmsprop_eps;
+    double mim_jump_prob, mim_jump_scale;
+    double rho_up, rho_down, delta_rho, delta_sigma;
+    double guided_alpha_base, guided_alpha_shock, guided_innovation_threshold;
+    double guide_strength_base, guide_strength_max, guide_innovation_threshold;
+    double rejuv_ksd_threshold, rejuv_prob, rejuv_blend;
+    double mu_process_var, mu_obs_var_scale, mu_min, mu_max;
+    double sigma_boost_threshold, sigma_boost_max;
+    double ksd_improvement_threshold;
+} svpfo_cfg;
+
+/* Scalar filter state carried between ticks (host fields of SVPFState and the
+ * device scalars of SVPFOptimizedState). */
+typedef struct svpfo_state {
+    int timestep;
+    int guide_initialized;
+    int stein_steps_used;
+    int pad_;
+    double ksd_prev;
+    double vol_prev;
+    double h_mean_prev;       /* d_h_mean_prev: mean of h after the last tick */
+    double guide_mean, guide_var;
+    double bw_sq_ema, return_ema, return_var;
+    double mu_state, mu_var;
+    double sigma_z_effective;
+} svpfo_state;
+
+/* Pre-drawn noise for one tick.  Any pointer may be NULL when the matching
+ * feature is off.  xi holds xi_rows rows of n Langevin normals; row s feeds
+ * Stein iteration s of the tick.  When z_pred is NULL the noise is drawn from
+ * the counter-based generator (svpfo_philox_*) keyed by (seed, filter, tick). */
+typedef struct svpfo_noise {
+    const float* z_pred;
+    const float* u_pred;
+    const float* xi;
+    int xi_rows;
+    int pad_;
+    const float* u_rej;
+    const float* z_rej;
+    uint64_t seed;
+    uint32_t filter_id;
+    uint32_t pad2_;
+} svpfo_noise;
+
+/* Per-tick outputs. */
+typedef struct svpfo_out {
+    double loglik;        /* reference loglik: logmeanexp of log_w at the last gradient evaluation */
+    double pred_loglik;   /* predictive loglik: logmeanexp of log p(y_t | h_pred) before any transport */
+    double vol;           /* mean exp(h/2) */
+    double h_mean;        /* mean h */
+    double ksd;           /* KSD of the last Stein iteration */
+    double bandwidth;     /* bw used by the transport this tick */
+    double bw_sq;         /* raw ell^2: median r^2 / EMA state / Silverman bw^2 (before the bw clamp) */
+    int steps;            /* Stein iterations executed */
+    int pad_;
+} svpfo_out;
+
+/* ---- closed-form helpers (each exported for the known-answer tests) ---- */
+double svpfo_student_t_const(double nu);
+double svpfo_implied_offset(double nu);
+double svpfo_log_w(double h, double y, double nu, double st_const);
+double svpfo_median_r2(const double* h, int n);
+float  svpfo_median_r2_f32(const float* h, int n);
+double svpfo_adaptive_beta(double ksd_prev, int anneal_idx, int n_anneal);
+int    svpfo_stein_budget(const svpfo_cfg* c, const svpfo_state* s);
+void   svpfo_ekf_update(const svpfo_cfg* c, svpfo_state* s, double y_t, double mu, double rho, double sigma_z);
+
+/* Mixture-prior + likelihood gradient for all particles (O(N^2)). */
+void svpfo_gradient(const svpfo_cfg* c, const double* h, const double* h_prev, double y_t, double y_prev,
+                    double mu, double sigma_z, double beta,
+                    double* grad, double* log_w, double* precond, double* curv);
+
+/* Stein operator phi and the reference KSD partial sums for all particles. */
+void svpfo_stein_phi(const svpfo_cfg* c, const double* h, const double* s, const double* curv,
+                     double inv_bw_sq, double* phi, double* ksd_partial);
+
+/* One full tick.  h, h_prev, v are length-n working arrays updated in place. */
+int svpfo_tick(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, double* v,
+               double y_t, double y_prev, const svpfo_noise* nz, svpfo_out* out);
+
+/* T ticks with pre-drawn noise laid out as z_pred[T][n], xi[T][xi_rows][n],
+ * u_pred/u_rej/z_rej[T][n] (NULL when unused).  Outputs are length T arrays
+ * (any may be NULL). */
+int svpfo_run(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, double* v,
+              int T, const float* y, float y_prev0,
+              const float* z_pred, const float* u_pred, const float* xi, int xi_rows,
+              const float* u_rej, const float* z_rej,
+              double* loglik, double* pred_loglik, double* vol, double* h_mean,
+              double* ksd, double* bw, int* steps);
+
+/* Counter-based generator: Philox4x32-10 (Salmon et al., SC'11). */
+void svpfo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double svpfo_u01_from_bits(uint32_t x);
+double svpfo_normal_from_bits(uint32_t a, uint32_t b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

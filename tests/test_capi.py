@@ -1,0 +1,68 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/svpf.h declares, and the ctypes mirror of SVPFState matches the C
+layout field by field."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from svpf_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "svpf.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = set()
+    for m in re.finditer(r"^\s*(?!static)[A-Za-z_][\w\s\*]*?\b(svpf_\w+)\s*\(", src, flags=re.M):
+        names.add(m.group(1))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_function():
+    L = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(_lib.EXPORTED)
+    assert L.svpf_version().startswith(b"svpf-mi355x")
+
+
+def test_tile_table():
+    L = _lib.load()
+    caps = [L.svpf_tile_capacity_query(t, 256) for t in range(8)]
+    assert caps == [64, 128, 256, 512, 1024, 2048, 256, 512]
+    assert L.svpf_tile_capacity_query(99, 256) == 0
+    assert L.svpf_lds_bytes_query(2048) >= 2048 * 20
+
+
+def test_state_layout_matches_header(tmp_path):
+    prog = tmp_path / "layout.c"
+    fields = [f for f, _ in _lib.SVPFState._fields_]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "svpf.h"', "int main(void) {",
+             'printf("size %zu\\n", sizeof(SVPFState));']
+    for f in fields:
+        lines.append(f'printf("{f} %zu\\n", offsetof(SVPFState, {f}));')
+    lines += ['printf("params %zu\\n", sizeof(SVPFParams));', 'printf("result %zu\\n", sizeof(SVPFResult));',
+              "return 0; }"]
+    prog.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include",
+                    str(prog), "-o", str(exe), "-lm"], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(line.split() for line in out if line.strip())
+    assert int(got["size"]) == ctypes.sizeof(_lib.SVPFState)
+    for f in fields:
+        assert int(got[f]) == getattr(_lib.SVPFState, f).offset, f
+    assert int(got["params"]) == ctypes.sizeof(_lib.SVPFParams)
+    assert int(got["result"]) == ctypes.sizeof(_lib.SVPFResult)
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(RuntimeError):
+        _lib.load(str(tmp_path / "nope.so"))

@@ -1,0 +1,216 @@
+"""Parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Tolerances: per-particle |dh| <= 1e-4 after one tick from identical state;
+filtered log-vol (h_mean) RMSE <= 1e-4 over whole trajectories; the median
+bandwidth selection is bit-exact against a float32 sort of the same particles.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from svpf_amd import config, data
+from svpf_amd.filter import Filter
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _hip():
+    return ctypes.CDLL("libamdhip64.so")
+
+
+def d2h(ptr, n):
+    out = np.zeros(n, dtype=np.float32)
+    rc = _hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ctypes.cast(ptr, ctypes.c_void_p).value),
+                          ctypes.c_size_t(4 * n), 2)
+    assert rc == 0
+    return out
+
+
+def dev(torch, a):
+    return None if a is None else torch.tensor(np.ascontiguousarray(a, dtype=np.float32), device="cuda")
+
+
+def run_gpu(torch, cfg, y, noise, h0, per_tick=True):
+    n = cfg["n"]
+    f = Filter(cfg)
+    f.set_particles(h0)
+    tz = {k: dev(torch, noise.get(k)) for k in ("z_pred", "xi", "u_pred", "u_rej", "z_rej")}
+    f.set_noise(tz["z_pred"], tz["xi"], tz["u_pred"], tz["u_rej"], tz["z_rej"])
+    rec = np.zeros((len(y), 8), dtype=np.float64)
+    yp = 0.0
+    for t in range(len(y)):
+        f.step(float(y[t]), yp)
+        rec[t] = f.last_outputs()
+        yp = float(y[t])
+    return f, rec
+
+
+def run_oracle(cfg, y, noise, h0):
+    o = oracle.OracleFilter(cfg, h0)
+    out = o.run(y, noise)
+    return o, out
+
+
+def compare(rec, out, tol_hmean=1e-4, tol_ll=1e-3, tol_vol=1e-4):
+    dh = rec[:, 2] - out["h_mean"]
+    assert np.sqrt(np.mean(dh ** 2)) <= tol_hmean, np.abs(dh).max()
+    assert np.max(np.abs(rec[:, 1] - out["vol"])) <= tol_vol * max(1.0, np.abs(out["vol"]).max())
+    assert np.max(np.abs(rec[:, 0] - out["loglik"])) <= tol_ll
+    assert np.array_equal(rec[:, 6].astype(int), out["steps"])
+
+
+def test_one_tick_from_identical_state(torch_cuda):
+    n = 512
+    cfg = config.plain_config(n=n)
+    rng = np.random.default_rng(11)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=5)
+    noise = data.predraw_noise(1, n, 5, seed=6)
+    y = np.array([0.42], dtype=np.float32)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0)
+    o, out = run_oracle(cfg, y, noise, h0)
+    h_gpu = f.particles()
+    assert np.max(np.abs(h_gpu - o.h)) <= 1e-4
+    assert np.max(np.abs(f.rmsprop_state() - o.v)) <= 1e-4
+    # median selection: bit-exact against a float32 sort of the device's predicted particles
+    hp = d2h(f.state.h_pred, n)
+    assert np.float32(rec[0, 7]) == oracle.median_r2_f32(hp)
+    del rng
+
+
+def test_plain_filter_trajectory_matches_golden(torch_cuda):
+    from golden.make_golden import plain_inputs
+
+    g = np.load(os.path.join(HERE, "golden", "plain_n512_t1000.npz"))
+    cfg, y, h_true, noise, h0 = plain_inputs()
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0)
+    out = {k: g[k] for k in ("h_mean", "vol", "loglik", "steps")}
+    compare(rec, out)
+    assert np.sqrt(np.mean((f.particles() - g["h_final"]) ** 2)) <= 1e-4
+
+
+FEATURE_CONFIGS = {
+    "reference_defaults": dict(),
+    "reference_defaults_exact_newton": dict(use_exact_gradient=1, lik_offset=0.25, use_newton=1),
+    "full_newton": dict(use_newton=1, use_full_newton=1, use_exact_gradient=1, stein_sign=1),
+    "plain_svgd_legacy": dict(use_newton=0, use_mim=0, use_guide=0, use_rejuvenation=0, stein_sign=0),
+    "silverman_paper": dict(bandwidth_mode=2, stein_sign=1, use_newton=1),
+    "median_guided": dict(bandwidth_mode=1, use_guided=1, use_adaptive_guide=1, use_newton=1),
+    "production": dict(nu=30.0, lik_offset=0.345, temperature=0.45, n_anneal_steps=5, mim_jump_prob=0.25,
+                       mim_jump_scale=9.0, rejuv_ksd_threshold=0.05, use_newton=1, use_full_newton=1,
+                       use_exact_gradient=1, stein_min_steps=8, stein_max_steps=16, use_guided=1,
+                       guided_alpha_shock=0.40, use_adaptive_guide=1, use_adaptive_mu=1,
+                       mu_obs_var_scale=11.0, mu_min=-4.0, use_adaptive_sigma=1, sigma_boost_threshold=0.95,
+                       sigma_boost_max=3.2),
+    "ksd_early_stop": dict(use_newton=1, use_ksd_early_stop=1, stein_min_steps=2, stein_max_steps=8,
+                           bandwidth_mode=1),
+    "fan_mode": dict(use_fan_mode=1, stein_sign=1),
+}
+
+
+@pytest.mark.parametrize("name", sorted(FEATURE_CONFIGS))
+def test_feature_configs_short_run(torch_cuda, name):
+    n, T = 300, 25
+    cfg = config.make_config(n=n, mu=-1.0, rho=0.97, sigma_z=0.15, gamma=-0.2, **FEATURE_CONFIGS[name])
+    y, _ = data.sv_series(T, mu=-1.0, rho=0.97, sigma_z=0.15, nu=5.0, seed=3)
+    y[12] *= 6.0  # a shock to trigger the adaptive branches
+    rows = cfg["stein_max_steps"]
+    noise = data.predraw_noise(T, n, rows, seed=4, mim=True, rejuv=True)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=5)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0)
+    o, out = run_oracle(cfg, y, noise, h0)
+    compare(rec, out, tol_hmean=1e-4, tol_ll=2e-3, tol_vol=2e-4)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 65, 400, 2048, 4096])
+def test_particle_counts(torch_cuda, n):
+    T = 6
+    cfg = config.plain_config(n=n)
+    y, _ = data.sv_series(T, seed=8)
+    noise = data.predraw_noise(T, n, 5, seed=9)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=10)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0)
+    o, out = run_oracle(cfg, y, noise, h0)
+    compare(rec, out, tol_ll=1e-3)
+    assert np.max(np.abs(f.particles() - o.h)) <= 2e-4
+
+
+def test_counter_generator_matches_oracle(torch_cuda):
+    n, T = 256, 20
+    cfg = config.plain_config(n=n)
+    y, _ = data.sv_series(T, seed=12)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=13)
+    f = Filter(cfg)
+    f.set_particles(h0)  # no pre-drawn noise: the counter generator, keyed by the default seed 42
+    o = oracle.OracleFilter(cfg, h0)
+    recs, outs = [], []
+    yp = 0.0
+    seed = 42
+    for t in range(T):
+        f.step(float(y[t]), yp)
+        recs.append(f.last_outputs())
+        outs.append(o.tick(float(y[t]), yp, seed=seed))
+        yp = float(y[t])
+    hm = np.array([r[2] for r in recs]) - np.array([q["h_mean"] for q in outs])
+    assert np.sqrt(np.mean(hm ** 2)) <= 1e-4
+    assert np.max(np.abs(f.particles() - o.h)) <= 5e-4
+
+
+def test_device_run_equals_per_tick_steps(torch_cuda):
+    torch = torch_cuda
+    n, T = 512, 40
+    cfg = config.plain_config(n=n)
+    y, _ = data.sv_series(T, seed=14)
+    noise = data.predraw_noise(T, n, 5, seed=15)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=16)
+    f1, rec = run_gpu(torch, cfg, y, noise, h0)
+    f2 = Filter(cfg)
+    f2.set_particles(h0)
+    f2.set_noise(dev(torch, noise["z_pred"]), dev(torch, noise["xi"]))
+    dy = dev(torch, y)
+    ll = torch.zeros(T, device="cuda")
+    vol = torch.zeros(T, device="cuda")
+    f2.run_device(dy, ll, vol)
+    # the device loop starts with y_prev = 0 like svpf_run_sequence (src/svpf_optimized_graph.cu:1414-1424)
+    assert np.array_equal(ll.cpu().numpy(), rec[:, 0].astype(np.float32))
+    assert np.array_equal(vol.cpu().numpy(), rec[:, 1].astype(np.float32))
+    assert np.array_equal(f1.particles(), f2.particles())
+    f3 = Filter(cfg)
+    f3.set_particles(h0)
+    f3.set_noise(dev(torch, noise["z_pred"]), dev(torch, noise["xi"]))
+    ll3, vol3 = f3.run_host(y)
+    assert np.array_equal(ll3, rec[:, 0].astype(np.float32))
+
+
+def test_unsupported_and_uninitialized_fail_loudly(torch_cuda):
+    from svpf_amd import _lib
+
+    cfg = config.plain_config(n=64)
+    f = Filter(cfg)
+    with pytest.raises(RuntimeError, match="not initialized"):
+        f.step(0.1, 0.0)
+    f.initialize(7)
+    f.state.use_heun = 1
+    with pytest.raises(RuntimeError, match="unsupported"):
+        f.step(0.1, 0.0)
+    f.state.use_heun = 0
+    f.step(0.1, 0.0)
+    assert f.L.svpf_last_error(f.sp) == _lib.SVPF_OK
+    assert not f.L.svpf_create(0, 5, 5.0, None)
+
+
+def test_graph_capture_replays(torch_cuda):
+    cfg = config.plain_config(n=128)
+    f = Filter(cfg)
+    f.initialize(3)
+    assert not f.L.svpf_graph_is_captured(f.sp)
+    f.step(0.1, 0.0)
+    assert f.L.svpf_graph_is_captured(f.sp)
+    a = [f.step(0.2, 0.1) for _ in range(3)]
+    f.L.svpf_graph_invalidate(f.sp)
+    assert not f.L.svpf_graph_is_captured(f.sp)
+    assert all(np.isfinite(x).all() for x in a)

@@ -1,0 +1,333 @@
+"""Pins the FP64 oracle (CPU only).
+
+The reference ships no golden vectors or known-answer tests for the step
+(its tests are print-only statistics), so each oracle function is checked
+against something independent of its own formula: published known-answer
+vectors (Philox), library routines (scipy's Student-t, digamma, brentq),
+numerical derivatives of the defining densities and kernels, brute-force
+selection, limits and invariants.  The two constants the reference states
+(implied offset at nu=5, psi(1/2)) are checked as known answers.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.optimize
+import scipy.special
+import scipy.stats
+
+import oracle
+from svpf_amd import config
+
+
+# ---------------------------------------------------------------- generator
+def test_philox_known_answers():
+    # Random123 kat_vectors, philox4x32 with 10 rounds
+    assert [hex(x) for x in oracle.philox([0, 0, 0, 0], [0, 0])] == [
+        "0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
+    assert [hex(x) for x in oracle.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)] == [
+        "0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
+    assert [hex(x) for x in oracle.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344],
+                                          [0xA4093822, 0x299F31D0])] == [
+        "0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+
+
+def test_uniform_and_normal_from_bits():
+    L = oracle.lib()
+    assert L.svpfo_u01_from_bits(0) == 2.0 ** -24
+    assert L.svpfo_u01_from_bits(0xFFFFFFFF) == 1.0
+    rng = np.random.default_rng(3)
+    a = rng.integers(0, 2 ** 32, size=200000, dtype=np.uint64)
+    b = rng.integers(0, 2 ** 32, size=200000, dtype=np.uint64)
+    z = np.array([L.svpfo_normal_from_bits(int(x), int(y)) for x, y in zip(a[:20000], b[:20000])])
+    assert abs(z.mean()) < 0.03 and abs(z.var() - 1.0) < 0.04
+    # Kolmogorov-Smirnov against N(0,1)
+    assert scipy.stats.kstest(z, "norm").pvalue > 1e-3
+
+
+# ---------------------------------------------------------------- closed forms
+@pytest.mark.parametrize("nu", [3.0, 4.0, 5.0, 8.0, 30.0])
+def test_student_t_const_is_log_density_at_zero(nu):
+    assert oracle.lib().svpfo_student_t_const(nu) == pytest.approx(scipy.stats.t.logpdf(0.0, nu), abs=1e-12)
+
+
+@pytest.mark.parametrize("nu", [4.0, 5.0, 30.0])
+def test_log_weight_is_student_t_log_density(nu):
+    L = oracle.lib()
+    c = L.svpfo_student_t_const(nu)
+    for h in (-3.0, -1.0, 0.5, 2.0):
+        for y in (-0.7, 0.01, 0.3, 2.5):
+            # the reference guards the variance with +1e-8 (src/svpf_opt_kernels.cu:478)
+            s = math.sqrt(math.exp(h) + 1e-8)
+            ref = scipy.stats.t.logpdf(y / s, nu) - h / 2
+            assert L.svpfo_log_w(h, y, nu, c) == pytest.approx(ref, abs=1e-9)
+
+
+def test_implied_offset_known_answer_and_asymptotic():
+    L = oracle.lib()
+    # reference constant: implied offset at nu=5 is about +1.057 (test/test_fully_fused_v2.cu:38)
+    assert L.svpfo_implied_offset(5.0) == pytest.approx(1.057, abs=5e-4)
+    # psi(1/2) used by the reference (src/svpf_optimized_graph.cu:128)
+    assert scipy.special.digamma(0.5) == pytest.approx(-1.9635100260214235, abs=1e-15)
+    for nu in (10.0, 30.0, 100.0):
+        exact = -(math.log(nu) + scipy.special.digamma(0.5) - scipy.special.digamma(nu / 2))
+        assert L.svpfo_implied_offset(nu) == pytest.approx(exact, abs=2e-5)
+    # Gaussian limit: -E[log chi^2_1] = 1.27 (include/svpf.cuh:346)
+    assert L.svpfo_implied_offset(1e6) == pytest.approx(1.2704, abs=1e-3)
+
+
+# ---------------------------------------------------------------- median bandwidth
+@pytest.mark.parametrize("n", [2, 3, 4, 7, 64, 257])
+def test_median_r2_brute_force(n):
+    rng = np.random.default_rng(n)
+    h = rng.normal(-1.0, 0.4, n)
+    if n > 4:
+        h[3] = h[1]  # ties
+    iu = np.triu_indices(n, 1)
+    r2 = ((h[:, None] - h[None, :]) ** 2)[iu]
+    k = (r2.size - 1) // 2
+    assert oracle.median_r2(h) == np.partition(r2, k)[k]
+    hf = h.astype(np.float32)
+    d = hf[:, None] - hf[None, :]
+    r2f = (d * d).astype(np.float32)[iu]
+    assert oracle.median_r2_f32(hf) == np.partition(r2f, k)[k]
+
+
+def test_median_r2_degenerate():
+    assert oracle.median_r2(np.array([0.3])) == 0.0
+    assert oracle.median_r2(np.full(10, -1.5)) == 0.0
+
+
+# ---------------------------------------------------------------- gradient
+def _cfg(**kw):
+    base = config.plain_config(n=kw.pop("n", 8))
+    base.update(kw)
+    return base
+
+
+def test_mixture_prior_gradient_is_derivative_of_log_mixture():
+    n = 9
+    rng = np.random.default_rng(1)
+    mu, rho, sig = -1.0, 0.95, 0.2
+    cfg = _cfg(n=n, mu=mu, rho=rho, sigma_z=sig, use_newton=0)
+    hp = rng.normal(mu, 0.3, n)
+    h = rng.normal(mu, 0.3, n)
+    g, lw, pc, cv = oracle.gradient(cfg, h, hp, 0.1, 0.0, mu, sig, 0.0)
+    m = mu + rho * (hp - mu)
+
+    def logmix(x):
+        return scipy.special.logsumexp(-(x - m) ** 2 / (2 * sig * sig))
+
+    eps = 1e-5
+    for j in range(n):
+        num = (logmix(h[j] + eps) - logmix(h[j] - eps)) / (2 * eps)
+        assert g[j] == pytest.approx(np.clip(num, -10, 10), abs=1e-6)
+
+
+def test_likelihood_gradient_and_curvature_are_derivatives_of_log_density():
+    n, nu = 5, 5.0
+    cfg = _cfg(n=n, nu=nu, sigma_z=1e3, use_newton=1, lik_offset=0.0, use_exact_gradient=1)
+    h = np.array([-2.0, -1.0, -0.3, 0.4, 1.2])
+    y = 0.35
+
+    def logp(x):
+        return scipy.stats.t.logpdf(y * math.exp(-x / 2), nu) - x / 2
+
+    g, lw, pc, cv = oracle.gradient(cfg, h, h, y, 0.0, -1.0, 1e3, 1.0)
+    eps = 1e-4
+    for j in range(n):
+        d1 = (logp(h[j] + eps) - logp(h[j] - eps)) / (2 * eps)
+        d2 = (logp(h[j] + eps) - 2 * logp(h[j]) + logp(h[j] - eps)) / eps ** 2
+        assert g[j] == pytest.approx(d1, abs=1e-5)
+        assert cv[j] == pytest.approx(np.clip(-d2 + 1e-6, 0.1, 100.0), abs=2e-4)
+        assert pc[j] == pytest.approx(0.7 * g[j] / cv[j], rel=1e-12)
+
+
+def test_surrogate_gradient():
+    cfg = _cfg(n=3, use_exact_gradient=0, lik_offset=0.45, sigma_z=1e3, use_newton=0)
+    h = np.array([-1.0, 0.0, 1.0])
+    y = 0.2
+    g, *_ = oracle.gradient(cfg, h, h, y, 0.0, -1.0, 1e3, 1.0)
+    m = -1.0 + cfg["rho"] * (h + 1.0)
+    # d/dh of log mixture prior + log-squared observation term -(log y^2 - h + off)^2 / (2 * 1.4)
+    for j in range(3):
+        f = lambda x: (scipy.special.logsumexp(-(x - m) ** 2 / (2e6))
+                       - ((math.log(y * y + 1e-10) - x + 0.45) ** 2) / (2 * 1.4))
+        num = (f(h[j] + 1e-5) - f(h[j] - 1e-5)) / 2e-5
+        assert g[j] == pytest.approx(num, abs=1e-7)
+
+
+# ---------------------------------------------------------------- Stein operator
+def _imq(a, b, l2):
+    return 1.0 / (1.0 + (a - b) ** 2 / l2)
+
+
+@pytest.mark.parametrize("sign", [0, 1])
+def test_repulsion_is_kernel_gradient(sign):
+    n = 6
+    rng = np.random.default_rng(7)
+    h = rng.normal(0, 0.5, n)
+    l2 = 0.3
+    cfg = _cfg(n=n, stein_sign=sign, use_newton=0)
+    phi, _ = oracle.stein_phi(cfg, h, np.zeros(n), np.ones(n), 1.0 / l2)
+    eps = 1e-6
+    for i in range(n):
+        # d/dx_j k(x_j, x_i)
+        num = np.mean([(_imq(h[j] + eps, h[i], l2) - _imq(h[j] - eps, h[i], l2)) / (2 * eps) for j in range(n)])
+        assert phi[i] == pytest.approx(num if sign == 1 else -num, abs=1e-8)
+
+
+def test_attraction_limits():
+    n = 5
+    s = np.array([0.5, -1.0, 2.0, 0.1, -0.3])
+    h = np.linspace(-1, 1, n)
+    cfg = _cfg(n=n, stein_sign=1, use_newton=0)
+    phi, _ = oracle.stein_phi(cfg, h, s, np.ones(n), 1e-12)  # ell -> inf: K -> 1, grad K -> 0
+    assert np.allclose(phi, s.mean(), atol=1e-9)
+    cfg1 = _cfg(n=1, use_newton=0)
+    phi1, _ = oracle.stein_phi(cfg1, np.array([0.3]), np.array([1.7]), np.ones(1), 4.0)
+    assert phi1[0] == pytest.approx(1.7, abs=1e-15)
+
+
+def test_paper_sign_repulsion_sums_to_zero():
+    n = 11
+    h = np.random.default_rng(2).normal(0, 1, n)
+    cfg = _cfg(n=n, stein_sign=1, use_newton=0)
+    phi, _ = oracle.stein_phi(cfg, h, np.zeros(n), np.ones(n), 2.0)
+    assert abs(phi.sum()) < 1e-12
+
+
+def test_local_newton_constant_curvature_scales_plain():
+    n = 7
+    rng = np.random.default_rng(4)
+    h = rng.normal(0, 0.5, n)
+    g = rng.normal(0, 1, n)
+    c = 3.7
+    plain = _cfg(n=n, stein_sign=1, use_newton=0)
+    newton = _cfg(n=n, stein_sign=1, use_newton=1, use_full_newton=0)
+    phi_p, _ = oracle.stein_phi(plain, h, 0.7 * g, np.ones(n), 1.5)
+    phi_n, _ = oracle.stein_phi(newton, h, 0.7 * g / c, np.full(n, c), 1.5)
+    assert np.allclose(phi_n, phi_p / c, atol=1e-12)
+
+
+def test_ksd_partials_against_numerical_stein_kernel():
+    """The reference's KSD uses +2K^2/l2 (4 d^2 K/l2 - 1) for d2k/dxdy, the
+    negative of the textbook term: reference = textbook - 2 d2k/dxdy."""
+    n = 6
+    rng = np.random.default_rng(5)
+    h = rng.normal(0, 0.6, n)
+    s = rng.normal(0, 1, n)
+    l2 = 0.4
+    cfg = _cfg(n=n, stein_sign=1, use_newton=0)
+    _, kp = oracle.stein_phi(cfg, h, s, np.ones(n), 1.0 / l2)
+    e = 1e-4
+    for i in range(n):
+        tot = 0.0
+        for j in range(n):
+            x, y = h[i], h[j]
+            k = _imq(x, y, l2)
+            kx = (_imq(x + e, y, l2) - _imq(x - e, y, l2)) / (2 * e)
+            ky = (_imq(x, y + e, l2) - _imq(x, y - e, l2)) / (2 * e)
+            kxy = (_imq(x + e, y + e, l2) - _imq(x + e, y - e, l2) - _imq(x - e, y + e, l2)
+                   + _imq(x - e, y - e, l2)) / (4 * e * e)
+            textbook = k * s[i] * s[j] + s[i] * ky + s[j] * kx + kxy
+            tot += textbook - 2 * kxy
+        assert kp[i] == pytest.approx(tot, abs=1e-5)
+
+
+# ---------------------------------------------------------------- scalar logic
+def test_adaptive_beta_known_values():
+    L = oracle.lib()
+    assert L.svpfo_adaptive_beta(0.6, 0, 3) == pytest.approx(0.3 * (0.5 + 0.5 / 3))
+    assert L.svpfo_adaptive_beta(0.01, 1, 3) == pytest.approx(0.8 * (0.5 + 0.5 * 2 / 3))
+    assert L.svpfo_adaptive_beta(0.275, 0, 2) == pytest.approx(0.55 * 0.75)
+    assert L.svpfo_adaptive_beta(0.275, 1, 2) == 1.0
+    # 0.3 * (0.5 + 0.5 * 0.1) = 0.165 stays above the 0.1 floor
+    assert L.svpfo_adaptive_beta(1e10, 0, 10) == pytest.approx(0.165)
+
+
+def test_stein_budget():
+    L = oracle.lib()
+    cfg = oracle.make_cfg(_cfg(stein_min_steps=4, stein_max_steps=12))
+    for ts, ksd, want in [(0, 0.0, 12), (9, 0.0, 12), (10, 0.0, 4), (10, 0.05, 4), (10, 0.5, 12),
+                          (10, 7.0, 12), (20, 0.275, 8)]:
+        st = oracle.State()
+        st.timestep = ts
+        st.ksd_prev = ksd
+        assert L.svpfo_stein_budget(ctypes_ref(cfg), ctypes_ref(st)) == want
+
+
+def ctypes_ref(x):
+    import ctypes
+
+    return ctypes.byref(x)
+
+
+def test_ekf_guide_converges_to_riccati_fixed_point():
+    L = oracle.lib()
+    cfg = _cfg(nu=5.0)
+    c = oracle.make_cfg(cfg)
+    st = oracle.make_state(cfg)
+    rho, sig, R = 0.97, 0.15, 4.93 + 2.0
+    for _ in range(400):
+        L.svpfo_ekf_update(ctypes_ref(c), ctypes_ref(st), 0.01, -1.0, rho, sig)
+
+    def f(P):
+        a = rho * rho * P + sig * sig
+        return a * R / (a + R) - P
+
+    Pstar = scipy.optimize.brentq(f, 1e-9, 10.0)
+    assert st.guide_var == pytest.approx(Pstar, rel=1e-7)
+
+
+# ---------------------------------------------------------------- whole tick
+def test_tick_zero_temperature_ignores_langevin_noise():
+    n = 16
+    cfg = config.plain_config(n=n, temperature=0.0)
+    rng = np.random.default_rng(0)
+    h0 = rng.normal(-1, 0.3, n)
+    z = rng.standard_normal(n).astype(np.float32)
+    outs = []
+    for seed in (1, 2):
+        xi = np.random.default_rng(seed).standard_normal((5, n)).astype(np.float32)
+        f = oracle.OracleFilter(cfg, h0)
+        f.tick(0.3, 0.0, z_pred=z, xi=xi)
+        outs.append(f.h.copy())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_tick_permutation_equivariant():
+    n = 24
+    cfg = config.plain_config(n=n)
+    rng = np.random.default_rng(0)
+    h0 = rng.normal(-1, 0.3, n)
+    z = rng.standard_normal(n).astype(np.float32)
+    xi = rng.standard_normal((5, n)).astype(np.float32)
+    perm = rng.permutation(n)
+    a = oracle.OracleFilter(cfg, h0)
+    oa = a.tick(0.3, 0.0, z_pred=z, xi=xi)
+    b = oracle.OracleFilter(cfg, h0[perm])
+    ob = b.tick(0.3, 0.0, z_pred=z[perm], xi=xi[:, perm])
+    assert np.allclose(a.h[perm], b.h, atol=1e-12)
+    for k in ("loglik", "vol", "h_mean", "ksd", "bandwidth"):
+        assert oa[k] == pytest.approx(ob[k], abs=1e-12)
+
+
+def test_tick_single_particle():
+    cfg = config.plain_config(n=1)
+    f = oracle.OracleFilter(cfg, np.array([-1.0]))
+    o = f.tick(0.2, 0.0, z_pred=np.array([0.5], np.float32), xi=np.zeros((5, 1), np.float32))
+    assert np.isfinite(f.h).all() and o["bandwidth"] == 0.01
+
+
+def test_counter_generator_mode_is_deterministic():
+    n = 32
+    cfg = config.plain_config(n=n)
+    h0 = np.random.default_rng(0).normal(-1, 0.3, n)
+    r = []
+    for _ in range(2):
+        f = oracle.OracleFilter(cfg, h0)
+        for t in range(3):
+            f.tick(0.2, 0.0, seed=99)
+        r.append(f.h.copy())
+    assert np.array_equal(r[0], r[1])
