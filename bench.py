@@ -1,0 +1,265 @@
+"""Benchmark of the SVPF step on one MI355X.
+
+Headline (value): filter-steps/s of the batched workload -- B=4096 independent
+filters, per-filter N ~ U{128, 256, 512, 1024, 2048}, per-filter theta
+(mu ~ U(-2,0), rho ~ U(.90,.995), sigma_z ~ U(.05,.4), nu ~ U(3,30)), each
+filter's own synthetic series, 5 Stein iterations, FP32.  A step is one tick
+of all B filters; K steps run in persistent tile-group launches with the
+inputs resident in HBM.
+
+Also reported: the same rate for the plain one-filter-per-launch path on the
+same GPU (per-tick C-ABI step of single filters, measured on a stratified
+subset and weighted to the workload's N mix) and the ratio; the single-filter
+N=512 device-loop microseconds per step and the interactive per-tick latency;
+the roofline of the dominant kernel (the N=2048 tile group) against the
+FP32 vector peak; the oracle timed on the host cores as CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "newton-stein-variational-particle-filter_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+PEAK_FP32 = 157.3e12  # MI355X FP32 vector, FLOP/s
+S_ITERS = 5
+FLOP_PER_PAIR = 23 * S_ITERS + 9  # per filter-step: N^2 (23 S + 9)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(B, T, seed=0):
+    from svpf_amd import config, data
+
+    n, mu, rho, sz, nu = data.batched_thetas(B, seed=seed)
+    cfgs, ys, h0s = [], np.zeros((B, T), np.float32), []
+    rng = np.random.default_rng(seed + 1)
+    for b in range(B):
+        cfgs.append(config.plain_config(n=int(n[b]), n_stein=S_ITERS, mu=mu[b], rho=rho[b], sigma_z=sz[b], nu=nu[b]))
+        # per-filter series: AR(1) log-vol with Student-t returns (vectorised over t)
+        e = rng.standard_normal(T)
+        h = np.empty(T)
+        h[0] = mu[b] + sz[b] / np.sqrt(1 - rho[b] ** 2) * e[0]
+        for t in range(1, T):
+            h[t] = mu[b] + rho[b] * (h[t - 1] - mu[b]) + sz[b] * e[t]
+        ys[b] = np.exp(h / 2) * rng.standard_t(nu[b], size=T)
+        sd = np.sqrt(sz[b] ** 2 / (1 - rho[b] ** 2 + 1e-6))
+        h0s.append(np.clip(mu[b] + sd * rng.standard_normal(int(n[b])), -15, 5).astype(np.float32))
+    return cfgs, ys, h0s, n
+
+
+def time_batched(torch, cfgs, ys, h0s, warmup, steps):
+    from svpf_amd.batched import Batch
+
+    bt = Batch(cfgs, ys, h0s)
+    torch.cuda.synchronize()
+    if warmup:
+        bt.run(0, warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bt.run(warmup, warmup + steps)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    st = bt.fstates()
+    assert (st["status"] == 0).all(), "kernel rejected a filter"
+    out = bt.outputs()[:, warmup:warmup + steps]
+    assert np.isfinite(out).all()
+    executed = out[:, :, 6].mean()
+    return wall, bt, executed
+
+
+def time_dominant_kernel(torch, cfgs, ys, h0s, n, warmup, steps):
+    """The N=2048 tile group alone, timed with HIP events on its stream."""
+    from svpf_amd.batched import Batch
+
+    idx = [b for b in range(len(cfgs)) if n[b] == n.max()]
+    bt = Batch([cfgs[b] for b in idx], ys[idx], [h0s[b] for b in idx])
+    assert bt.n_groups == 1
+    if warmup:
+        bt.run(0, warmup)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    bt.run(warmup, warmup + steps)
+    e1.record(s)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    flops = float(len(idx)) * float(n.max()) ** 2 * FLOP_PER_PAIR * steps
+    return ms, flops, len(idx), bt.groups
+
+
+def time_plain(torch, cfgs, ys, h0s, n, ticks=8):
+    """Plain path: one filter per launch per tick through svpf_step_graph."""
+    from svpf_amd.filter import Filter
+
+    per_bucket = {}
+    for N in sorted(set(n.tolist())):
+        b = int(np.nonzero(n == N)[0][0])
+        f = Filter(cfgs[b])
+        f.set_particles(h0s[b])
+        f.step(float(ys[b, 0]), 0.0)  # capture the tick graph
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(1, ticks + 1):
+            f.step(float(ys[b, t]), float(ys[b, t - 1]))
+        per_bucket[N] = (time.perf_counter() - t0) / ticks
+        f.close()
+    # expected seconds per filter-step under the workload's N mix
+    mix = np.mean([per_bucket[int(N)] for N in n])
+    return 1.0 / mix, per_bucket
+
+
+def time_single(torch, T=1000, warm=50):
+    """Single filter N=512: device loop over T ticks, and the per-tick interactive latency."""
+    from svpf_amd import config, data
+    from svpf_amd.filter import Filter
+
+    cfg = config.plain_config(n=512)
+    y, _ = data.sv_series(T + warm, seed=0)
+    h0 = data.initial_particles(512, -1.0, 0.97, 0.15, seed=2)
+    f = Filter(cfg)
+    f.set_particles(h0)
+    dy = torch.tensor(y, device="cuda")
+    f.run_device(dy[:warm])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f.run_device(dy[warm:])
+    torch.cuda.synchronize()
+    us_device = (time.perf_counter() - t0) / T * 1e6
+    lat = []
+    yp = float(y[warm - 1])
+    for t in range(warm, warm + 200):
+        t1 = time.perf_counter()
+        f.step(float(y[t]), yp)
+        lat.append(time.perf_counter() - t1)
+        yp = float(y[t])
+    f.close()
+    lat = np.array(lat[10:]) * 1e6
+    return us_device, float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+def cpu_baseline(cfgs, ys, h0s, n, ticks=3):
+    """FP64 oracle, single thread, one filter per N bucket for `ticks` ticks."""
+    import oracle
+
+    per_bucket = {}
+    for N in sorted(set(n.tolist())):
+        b = int(np.nonzero(n == N)[0][0])
+        f = oracle.OracleFilter(cfgs[b], h0s[b])
+        t0 = time.perf_counter()
+        yp = 0.0
+        for t in range(ticks):
+            f.tick(float(ys[b, t]), yp, seed=b, filter_id=b)
+            yp = float(ys[b, t])
+        per_bucket[N] = (time.perf_counter() - t0) / ticks
+    mix = np.mean([per_bucket[int(N)] for N in n])
+    return 1.0 / mix, per_bucket
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--B", type=int, default=4096)
+    ap.add_argument("--no-extras", action="store_true", help="headline only")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: filters are independent and this build runs on one MI355X; "
+              "--gpus above 1 is not supported.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    from svpf_amd import _lib
+
+    _lib.load()
+    B, K, W = args.B, args.steps, args.warmup
+    log(f"building workload B={B} T={W + K}")
+    cfgs, ys, h0s, n = build_workload(B, W + K + 1)
+    log("batched run")
+    wall, bt, executed = time_batched(torch, cfgs, ys, h0s, W, K)
+    value = B * K / wall
+    ms_per_step = wall / K * 1e3
+    log(f"batched: {value:.1f} filter-steps/s ({ms_per_step:.3f} ms/step)")
+
+    log("dominant kernel (N=2048 group)")
+    ms_dom, flops_dom, nf_dom, groups = time_dominant_kernel(torch, cfgs, ys, h0s, n, W, K)
+    achieved = flops_dom / (ms_dom * 1e-3)
+    total_flops = float((n.astype(np.float64) ** 2).sum()) * FLOP_PER_PAIR * K
+
+    extras = {}
+    if not args.no_extras:
+        log("plain one-filter-per-launch path")
+        plain_rate, plain_bucket = time_plain(torch, cfgs, ys, h0s, n)
+        log("single filter N=512")
+        us_dev, p50, p99 = time_single(torch)
+        extras = {
+            "plain_filter_steps_per_s": plain_rate,
+            "batched_vs_plain": value / plain_rate,
+            "plain_s_per_step_by_n": {str(k): v for k, v in plain_bucket.items()},
+            "single_n512_us_per_step_device_loop": us_dev,
+            "single_n512_tick_latency_us_p50": p50,
+            "single_n512_tick_latency_us_p99": p99,
+            "fwd_bwd_us_per_step": None,
+        }
+    log("cpu baseline (oracle)")
+    cpu_rate, cpu_bucket = cpu_baseline(cfgs, ys, h0s, n)
+
+    rec = {
+        "metric": "filter-steps/s at B=4096 batched vs plain; us/step at N=512; fwd+bwd us/step",
+        "value": value,
+        "unit": "filter-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched SVPF, B={B} filters, N~U{{128..2048}}, per-filter theta and series, "
+                        f"S={S_ITERS} Stein iterations, median IMQ bandwidth, local Newton, SVLD",
+            "B": B, "stein_iters": S_ITERS, "executed_iters_mean": float(executed),
+            "particles_total": int(n.sum()), "tile_groups": int(bt.n_groups),
+        },
+        "achieved_tflops_whole_batch": total_flops / wall / 1e12,
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved / 1e12,
+            "peak": PEAK_FP32 / 1e12,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32,
+            "traffic": None,
+            "kernel": f"svpf_batched_kernel<256,8,1> (N=2048 group, {nf_dom} filters, grid {int(groups['grid'][0])})",
+            "kernel_ms": ms_dom,
+            "flops_per_launch": flops_dom,
+        },
+        "cpu_baseline": {
+            "value": cpu_rate,
+            "unit": "filter-steps/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": "FP64 oracle, one filter per N bucket {128..2048} x 3 ticks, weighted to the workload N mix",
+            "s_per_step_by_n": {str(k): v for k, v in cpu_bucket.items()},
+        },
+    }
+    rec.update(extras)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -86,6 +86,30 @@ struct svpf_impl {
 
 namespace {
 
+// Worker streams (default priority) and events for the batched fork/join,
+// created once per process on the current device.  Calls that overlap from
+// several host threads share them and are not supported.
+constexpr int kMaxForkStreams = 8;
+struct ForkJoin {
+    hipStream_t streams[kMaxForkStreams];
+    hipEvent_t fork;
+    hipEvent_t join[kMaxForkStreams];
+};
+
+ForkJoin* fork_join() {
+    static ForkJoin fj;
+    static int state = 0;  // 0 untried, 1 ready, -1 failed
+    if (state == 0) {
+        bool ok = hipEventCreateWithFlags(&fj.fork, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; ok && i < kMaxForkStreams; ++i) {
+            ok = hipStreamCreateWithFlags(&fj.streams[i], hipStreamNonBlocking) == hipSuccess;
+            ok = ok && hipEventCreateWithFlags(&fj.join[i], hipEventDisableTiming) == hipSuccess;
+        }
+        state = ok ? 1 : -1;
+    }
+    return state == 1 ? &fj : nullptr;
+}
+
 int set_err(SVPFState* s, int code) {
     if (s && s->impl) s->impl->last_error = code;
     return code;
@@ -831,18 +855,42 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
         !group_tile || !group_bs || !group_grid || !group_nmax || n_groups < 0 || t1 < t0)
         return SVPF_ERR_INVALID_ARGUMENT;
     if (svpf_kernels_init() != hipSuccess) return SVPF_ERR_HIP;
+    int used = 0;
     for (int g = 0; g < n_groups; ++g) {
         int nf = group_start[g + 1] - group_start[g];
         if (nf <= 0) continue;
         int cap = svpf_tile_capacity(group_tile[g], group_bs[g]);
         if (cap <= 0 || group_nmax[g] > cap || group_grid[g] <= 0) return SVPF_ERR_INVALID_ARGUMENT;
         if (svpf_lds_bytes(group_nmax[g]) > 159 * 1024) return SVPF_ERR_UNSUPPORTED;
+        ++used;
+    }
+    if (used > kMaxForkStreams) return SVPF_ERR_INVALID_ARGUMENT;
+    // Tile groups run concurrently: fork from `stream` onto worker streams and
+    // join back, so the call stays one ordered operation on `stream` (and is
+    // capturable into a graph).
+    ForkJoin* fj = used > 1 ? fork_join() : nullptr;
+    if (used > 1 && !fj) return SVPF_ERR_HIP;
+    if (fj && hipEventRecord(fj->fork, stream) != hipSuccess) return SVPF_ERR_HIP;
+    int k = 0;
+    for (int g = 0; g < n_groups; ++g) {
+        int nf = group_start[g + 1] - group_start[g];
+        if (nf <= 0) continue;
+        hipStream_t s = stream;
+        if (fj) {
+            s = fj->streams[k];
+            if (hipStreamWaitEvent(s, fj->fork, 0) != hipSuccess) return SVPF_ERR_HIP;
+        }
         hipError_t e = svpf_launch_batched(group_bs[g], group_tile[g], group_grid[g],
                                            static_cast<const svpf_fparams*>(d_params),
                                            static_cast<svpf_fstate*>(d_fstate), static_cast<const svpf_fdesc*>(d_desc),
                                            d_pool_h, d_pool_hp, d_pool_v, d_order + group_start[g], nf,
-                                           d_queues ? d_queues + g : nullptr, group_nmax[g], t0, t1, stream);
+                                           d_queues ? d_queues + g : nullptr, group_nmax[g], t0, t1, s);
         if (e != hipSuccess) return SVPF_ERR_HIP;
+        if (fj) {
+            if (hipEventRecord(fj->join[k], s) != hipSuccess) return SVPF_ERR_HIP;
+            if (hipStreamWaitEvent(stream, fj->join[k], 0) != hipSuccess) return SVPF_ERR_HIP;
+        }
+        ++k;
     }
     return SVPF_OK;
 }

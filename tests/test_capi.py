@@ -66,3 +66,37 @@ def test_state_layout_matches_header(tmp_path):
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError):
         _lib.load(str(tmp_path / "nope.so"))
+
+
+def test_record_layouts_match_c(tmp_path):
+    """numpy record dtypes of the batched API against csrc/svpf_types.h."""
+    from svpf_amd import batched
+
+    csrc = os.path.join(ROOT, "newton-stein-variational-particle-filter_amd", "csrc")
+    checks = [("svpf_fparams", batched.FPARAMS_DTYPE), ("svpf_fstate", batched.FSTATE_DTYPE),
+              ("svpf_fdesc", batched.FDESC_DTYPE)]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "svpf_types.h"', "int main(void) {"]
+    for cname, dt in checks:
+        lines.append(f'printf("{cname}.size %zu\\n", sizeof({cname}));')
+        for f in dt.names:
+            if f.startswith("pad"):
+                continue
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append('printf("flags %u %u %u\\n", (unsigned)SVPF_F_EXACT_GRAD, (unsigned)SVPF_F_KSD_EARLY_STOP,'
+                 ' (unsigned)SVPF_F_PAPER_SIGN);')
+    lines.append("return 0; }")
+    prog = tmp_path / "rec.c"
+    prog.write_text("\n".join(lines))
+    exe = tmp_path / "rec"
+    subprocess.run(["gcc", "-I", csrc, str(prog), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.strip().split("\n")
+    got = {}
+    for line in out:
+        k, *v = line.split()
+        got[k] = v
+    for cname, dt in checks:
+        assert int(got[f"{cname}.size"][0]) == dt.itemsize
+        for f in dt.names:
+            if not f.startswith("pad"):
+                assert int(got[f"{cname}.{f}"][0]) == dt.fields[f][1], (cname, f)
+    assert [int(x) for x in got["flags"]] == [batched.F_EXACT_GRAD, batched.F_KSD_EARLY_STOP, batched.F_PAPER_SIGN]
