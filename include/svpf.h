@@ -57,6 +57,7 @@ extern "C" {
 #define SVPF_ERR_HIP 3
 #define SVPF_ERR_NOT_INITIALIZED 4
 #define SVPF_ERR_OUT_OF_MEMORY 5
+#define SVPF_ERR_TIMEOUT 6 /* a cooperative kernel's cross-workgroup exchange timed out */
 
 /* include/svpf.cuh:257-262 */
 typedef struct {
@@ -171,7 +172,7 @@ typedef struct {
     float smooth_h_var[SVPF_SMOOTH_MAX_LAG];
     float smooth_y[SVPF_SMOOTH_MAX_LAG];
 
-    int use_persistent_kernel;
+    int use_persistent_kernel; /* 1: cooperative multi-workgroup kernel (default for n >= 256) */
     int persistent_kernel_supported;
     int use_heun;      /* rejected: SVPF_ERR_UNSUPPORTED */
     int use_two_factor; /* rejected: SVPF_ERR_UNSUPPORTED */
@@ -229,8 +230,10 @@ float svpf_get_ess(const SVPFState* state);
 /* src/svpf_optimized_graph.cu:1490-1493 (defined there, never declared) */
 void svpf_get_ksd_stats(const SVPFState* state, float* ksd_out, int* steps_used_out);
 
-/* No reference counterpart: status of the last call on this state (SVPF_OK ...). */
+/* No reference counterpart: status of the last call on this state (SVPF_OK ...)
+ * and a readable description (the HIP error string for SVPF_ERR_HIP). */
 int svpf_last_error(const SVPFState* state);
+const char* svpf_last_error_string(const SVPFState* state);
 
 /* No reference counterpart: parity hooks for identical-input comparisons.
  * svpf_set_particles uploads h, h_prev and the RMSProp second moment (any may
@@ -250,6 +253,14 @@ int svpf_set_scalar_state(SVPFState* state, float h_mean_prev, float bw_sq_ema, 
 /* No reference counterpart: per-tick diagnostics of the last tick
  * (loglik, vol, h_mean, ksd, bandwidth, predictive loglik, steps, ell^2). */
 int svpf_get_last_outputs(const SVPFState* state, float out8[8]);
+
+/* No reference counterpart: shape of the cooperative multi-workgroup kernel
+ * that serves one filter when use_persistent_kernel is set (the default for
+ * n_particles >= 256).  lanes_per_row in {1,2,4,8,16,32} gives 256 /
+ * lanes_per_row particles per workgroup; xcd_stride 8 places all workgroups
+ * on one XCD, 1 spreads them.  Pass -1 for the defaults. */
+int svpf_set_coop_shape(SVPFState* state, int lanes_per_row, int xcd_stride);
+int svpf_get_coop_shape(const SVPFState* state, int* lanes_per_row, int* workgroups, int* xcd_stride);
 
 /* include/svpf.cuh:626-632 */
 static inline void svpf_set_stein_sign_mode(SVPFState* state, int mode) {

@@ -62,10 +62,20 @@ struct svpf_impl {
     uint64_t seed = 42;
     bool initialized = false;
     int last_error = SVPF_OK;
+    hipError_t last_hip = hipSuccess;  // HIP status behind the last SVPF_ERR_HIP
+    int last_status = 0;               // kernel-reported status behind the last kernel error
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool captured = false;
     float out_last[8] = {0};
+    // cooperative multi-workgroup path
+    bool coop_ok = false;
+    int coop_cs = 8, coop_G = 1, coop_stride = 8;
+    uint64_t* xbuf = nullptr;  // tagged exchange words
+    uint32_t epoch = 0;
+    float med_r2 = 0.0f;  // last median pairwise r^2 (a search seed, not a result)
+    bool captured_coop = false;
+    bool coop_graph_failed = false;
 
     svpf_fparams* hp() { return reinterpret_cast<svpf_fparams*>(hblock + kBlockParams); }
     svpf_fstate* hs() { return reinterpret_cast<svpf_fstate*>(hblock + kBlockFstate); }
@@ -108,6 +118,30 @@ ForkJoin* fork_join() {
         state = ok ? 1 : -1;
     }
     return state == 1 ? &fj : nullptr;
+}
+
+// Cooperative shape for n particles: rows per workgroup = 256 / cs, G = ceil(n / rows).
+// Default: about 16 workgroups (rows between 8 and 256), all on one XCD (stride 8).
+bool choose_coop_shape(svpf_impl* I, int cs, int stride) {
+    const int n = I->n;
+    if (cs <= 0) {
+        int rows = 8;
+        while (rows < 256 && rows * 16 < n) rows <<= 1;
+        cs = 256 / rows;
+    }
+    if (cs != 1 && cs != 2 && cs != 4 && cs != 8 && cs != 16 && cs != 32) return false;
+    if (stride <= 0) stride = 8;
+    if (stride != 1 && stride != 8) return false;
+    const int rows = 256 / cs;
+    const int G = (n + rows - 1) / rows;
+    // partial sums of all workgroups (G x 8 words) are gathered into an n-float LDS array
+    const bool fits = svpf_coop_lds_bytes(n) <= 159 * 1024 && G * 8 <= n && G <= n / 8 + 2 && G * stride <= 2048;
+    if (!fits) return false;
+    I->coop_cs = cs;
+    I->coop_G = G;
+    I->coop_stride = stride;
+    I->coop_ok = true;
+    return true;
 }
 
 int set_err(SVPFState* s, int code) {
@@ -227,6 +261,8 @@ void pack_fstate(const SVPFState* s, float y_prev, svpf_fstate* o) {
     o->mu_var = s->mu_var;
     o->sigma_z_eff = s->sigma_z_effective;
     o->y_prev = y_prev;
+    o->epoch = I->epoch;
+    o->med_r2 = I->med_r2;
 }
 
 void unpack_fstate(SVPFState* s, const svpf_fstate* o) {
@@ -245,6 +281,8 @@ void unpack_fstate(SVPFState* s, const svpf_fstate* o) {
     I->bw_sq_ema = o->bw_sq_ema;
     I->return_ema = o->return_ema;
     I->return_var = o->return_var;
+    I->epoch = o->epoch;
+    I->med_r2 = o->med_r2;
 }
 
 void fill_desc(SVPFState* s, svpf_fdesc* d, const float* y, float* out, int tick0) {
@@ -282,12 +320,27 @@ int check_supported(SVPFState* s) {
     return SVPF_OK;
 }
 
+// The cooperative multi-workgroup kernel serves one filter when the state asks
+// for it (use_persistent_kernel) and the features it implements suffice.
+bool use_coop(const SVPFState* s) {
+    const svpf_impl* I = s->impl;
+    return s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop;
+}
+
+hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
+    svpf_impl* I = s->impl;
+    if (use_coop(s))
+        return svpf_launch_coop(I->coop_cs, I->coop_G, I->coop_stride, I->dp(), I->ds(), I->dd(), I->d_h(),
+                                I->d_hp(), I->d_v(), I->xbuf, I->n, t0, t1, 1, cs);
+    return svpf_launch_batched(I->bs, I->tile, 1, I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->dorder,
+                               1, nullptr, I->n, t0, t1, cs);
+}
+
 hipError_t enqueue_tick(SVPFState* s, hipStream_t cs) {
     svpf_impl* I = s->impl;
     hipError_t e = hipMemcpyAsync(I->dblock, I->hblock, kUploadBytes, hipMemcpyHostToDevice, cs);
     if (e != hipSuccess) return e;
-    e = svpf_launch_batched(I->bs, I->tile, 1, I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(),
-                            I->dorder, 1, nullptr, I->n, 0, 1, cs);
+    e = launch_step_kernel(s, 0, 1, cs);
     if (e != hipSuccess) return e;
     // fstate (128 B) and the output record (32 B) are adjacent: one copy back.
     return hipMemcpyAsync(I->hblock + kBlockFstate, I->dblock + kBlockFstate, kBlockOut + 32 - kBlockFstate,
@@ -373,6 +426,9 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     I->seed = saved;
     hipStream_t cs = s->stream;
     hipError_t e = hipSuccess;
+    const bool coop = use_coop(s);
+    if (I->captured && I->captured_coop != coop) svpf_graph_invalidate(s);  // path changed since capture
+    if (use_graph && coop && I->coop_graph_failed) use_graph = false;
     if (use_graph) {
         if (!I->captured) {
             hipStream_t cap = nullptr;
@@ -387,23 +443,31 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
                 if (e == hipSuccess) {
                     I->graph = g;
                     I->captured = true;
+                    I->captured_coop = coop;
                 } else if (g) {
                     hipGraphDestroy(g);
                 }
             }
             if (cap) hipStreamDestroy(cap);
+            if (e != hipSuccess && coop) {
+                // cooperative launches that cannot be captured run as plain stream work
+                (void)hipGetLastError();
+                I->coop_graph_failed = true;
+                use_graph = false;
+                e = hipSuccess;
+            }
         }
-        if (e == hipSuccess) e = hipGraphLaunch(I->exec, cs);
-    } else {
-        e = enqueue_tick(s, cs);
     }
+    if (e == hipSuccess) e = use_graph ? hipGraphLaunch(I->exec, cs) : enqueue_tick(s, cs);
     if (e == hipSuccess) e = hipStreamSynchronize(cs);
     if (e != hipSuccess) {
+        I->last_hip = e;
         set_err(s, SVPF_ERR_HIP);
         return;
     }
     if (I->hs()->status) {
-        set_err(s, SVPF_ERR_UNSUPPORTED);
+        I->last_status = I->hs()->status;
+        set_err(s, I->hs()->status == 2 ? SVPF_ERR_TIMEOUT : SVPF_ERR_UNSUPPORTED);
         return;
     }
     unpack_fstate(s, I->hs());
@@ -429,15 +493,19 @@ int run_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, flo
     fill_desc(s, I->hd(), d_obs, I->dseq_out, s->timestep);
     hipStream_t cs = s->stream;
     hipError_t e = hipMemcpyAsync(I->dblock, I->hblock, kUploadBytes, hipMemcpyHostToDevice, cs);
-    if (e == hipSuccess)
-        e = svpf_launch_batched(I->bs, I->tile, 1, I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(),
-                                I->dorder, 1, nullptr, I->n, 0, T, cs);
+    if (e == hipSuccess) e = launch_step_kernel(s, 0, T, cs);
     if (e == hipSuccess)
         e = hipMemcpyAsync(I->hblock + kBlockFstate, I->dblock + kBlockFstate, sizeof(svpf_fstate),
                            hipMemcpyDeviceToHost, cs);
     if (e == hipSuccess) e = hipStreamSynchronize(cs);
-    if (e != hipSuccess) return SVPF_ERR_HIP;
-    if (I->hs()->status) return SVPF_ERR_UNSUPPORTED;
+    if (e != hipSuccess) {
+        I->last_hip = e;
+        return SVPF_ERR_HIP;
+    }
+    if (I->hs()->status) {
+        I->last_status = I->hs()->status;
+        return I->hs()->status == 2 ? SVPF_ERR_TIMEOUT : SVPF_ERR_UNSUPPORTED;
+    }
     unpack_fstate(s, I->hs());
     return SVPF_OK;
 }
@@ -466,13 +534,19 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
     ok = ok && hipHostMalloc(&I->hblock, kBlockBytes, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipMalloc(&I->dpool, 5 * (size_t)n_particles * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&I->dorder, sizeof(int)) == hipSuccess;
+    // exchange words of the cooperative path: [2][4][n] records, [2][n] final positions,
+    // [2][G][8] partials with G <= n / 8 + 1
+    const size_t xwords = 10 * (size_t)n_particles + 16 * ((size_t)n_particles / 8 + 2);
+    ok = ok && hipMalloc(&I->xbuf, xwords * sizeof(uint64_t)) == hipSuccess;
     if (ok) {
         std::memset(I->hblock, 0, kBlockBytes);
         int zero = 0;
         ok = hipMemcpy(I->dorder, &zero, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
         ok = ok && hipMemset(I->dpool, 0, 5 * (size_t)n_particles * sizeof(float)) == hipSuccess;
         ok = ok && hipMemset(I->dblock, 0, kBlockBytes) == hipSuccess;
+        ok = ok && hipMemset(I->xbuf, 0, xwords * sizeof(uint64_t)) == hipSuccess;
     }
+    if (ok) choose_coop_shape(I, -1, -1);
     if (!ok) {
         svpf_destroy(s);
         return nullptr;
@@ -560,7 +634,7 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
         s->smooth_y[i] = 0.0f;
     }
     s->smooth_head = 0;
-    s->use_persistent_kernel = 0;
+    s->use_persistent_kernel = (n_particles >= 256 && I->coop_ok) ? 1 : 0;
     s->persistent_kernel_supported = 1;
     s->bandwidth_mode = SVPF_BANDWIDTH_REFERENCE_EMA;
     s->use_linear_beta = 0;
@@ -578,6 +652,7 @@ void svpf_destroy(SVPFState* s) {
         if (I->hblock) hipHostFree(I->hblock);
         if (I->dpool) hipFree(I->dpool);
         if (I->dorder) hipFree(I->dorder);
+        if (I->xbuf) hipFree(I->xbuf);
         if (I->dseq_out) hipFree(I->dseq_out);
         if (I->dseq_y) hipFree(I->dseq_y);
         delete I;
@@ -614,6 +689,7 @@ void svpf_initialize(SVPFState* s, const SVPFParams* p, unsigned long long seed)
     if (s->use_adaptive_sigma) s->sigma_z_effective = p->sigma_z;
     s->ksd_prev = 1e10f;
     s->stein_steps_used = s->n_stein_steps;
+    I->med_r2 = 0.0f;
     svpf_graph_invalidate(s);
     I->initialized = true;
     set_err(s, SVPF_OK);
@@ -792,6 +868,21 @@ void svpf_get_ksd_stats(const SVPFState* s, float* ksd_out, int* steps_used_out)
 
 int svpf_last_error(const SVPFState* s) { return (s && s->impl) ? s->impl->last_error : SVPF_ERR_INVALID_ARGUMENT; }
 
+const char* svpf_last_error_string(const SVPFState* s) {
+    if (!s || !s->impl) return "invalid state";
+    const svpf_impl* I = s->impl;
+    switch (I->last_error) {
+        case SVPF_OK: return "ok";
+        case SVPF_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case SVPF_ERR_UNSUPPORTED: return "unsupported feature";
+        case SVPF_ERR_HIP: return hipGetErrorString(I->last_hip);
+        case SVPF_ERR_NOT_INITIALIZED: return "filter not initialized";
+        case SVPF_ERR_OUT_OF_MEMORY: return "out of memory";
+        case SVPF_ERR_TIMEOUT: return "cooperative exchange timed out";
+        default: return "unknown";
+    }
+}
+
 int svpf_set_particles(SVPFState* s, const float* h, const float* h_prev, const float* v) {
     if (!s || !s->impl || !h) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
     svpf_impl* I = s->impl;
@@ -892,6 +983,29 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
         }
         ++k;
     }
+    return SVPF_OK;
+}
+
+int svpf_set_coop_shape(SVPFState* s, int lanes_per_row, int xcd_stride) {
+    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    svpf_impl* I = s->impl;
+    svpf_impl trial = {};
+    trial.n = I->n;
+    if (!choose_coop_shape(&trial, lanes_per_row, xcd_stride)) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+    I->coop_cs = trial.coop_cs;
+    I->coop_G = trial.coop_G;
+    I->coop_stride = trial.coop_stride;
+    I->coop_ok = true;
+    svpf_graph_invalidate(s);
+    return set_err(s, SVPF_OK);
+}
+
+int svpf_get_coop_shape(const SVPFState* s, int* lanes_per_row, int* workgroups, int* xcd_stride) {
+    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    const svpf_impl* I = s->impl;
+    if (lanes_per_row) *lanes_per_row = I->coop_ok ? I->coop_cs : 0;
+    if (workgroups) *workgroups = I->coop_ok ? I->coop_G : 0;
+    if (xcd_stride) *xcd_stride = I->coop_ok ? I->coop_stride : 0;
     return SVPF_OK;
 }
 

@@ -10,9 +10,26 @@
 
 #define SVPF_DEV __device__ __forceinline__
 
+// Phase timestamps for latency analysis (off unless built with -DSVPF_TRACE):
+// thread 0 of block 0 stores the 100 MHz wall clock into slot k of the
+// tick's 64-slot row.
+#ifdef SVPF_TRACE
+extern __device__ unsigned long long svpf_trace_buf[256 * 64];
+#define SVPF_TRACE_POINT(tick, k)                                                         \
+    do {                                                                                  \
+        if (threadIdx.x == 0 && blockIdx.x == 0)                                          \
+            svpf_trace_buf[(((unsigned)(tick)) & 255u) * 64u + ((k) & 63)] = wall_clock64(); \
+    } while (0)
+#else
+#define SVPF_TRACE_POINT(tick, k) \
+    do {                          \
+    } while (0)
+#endif
+
 namespace svpf {
 
 constexpr int kWave = 64;
+constexpr int kMedianCands = 32;  // candidate pairs gathered for the final rank selection
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
@@ -69,40 +86,97 @@ struct CounterRng {
 };
 
 // ---------------------------------------------------------------------------
-// Reductions (wave64 butterflies, then one LDS hop across waves).
+// Reductions.  A wave64 reduction is a DPP inclusive scan -- row_shr 1/2/4/8
+// inside each 16-lane row, then row_bcast 15 and 31 across rows -- after
+// which lane 63 holds the total, broadcast with readlane.  Lanes whose DPP
+// source lies outside the row keep the identity element.  All 64 lanes must
+// be active.  Workgroup reductions add one LDS hop across waves.
 // ---------------------------------------------------------------------------
+#define SVPF_DPP_F(OLD, V, CTRL, RM) \
+    __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(OLD), __float_as_int(V), CTRL, RM, 0xF, false))
+#define SVPF_DPP_U(OLD, V, CTRL, RM) ((uint32_t)__builtin_amdgcn_update_dpp((int)(OLD), (int)(V), CTRL, RM, 0xF, false))
+
 SVPF_DEV float wave_sum(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += SVPF_DPP_F(0.0f, v, 0x111, 0xF);  // row_shr:1
+    v += SVPF_DPP_F(0.0f, v, 0x112, 0xF);  // row_shr:2
+    v += SVPF_DPP_F(0.0f, v, 0x114, 0xF);  // row_shr:4
+    v += SVPF_DPP_F(0.0f, v, 0x118, 0xF);  // row_shr:8
+    v += SVPF_DPP_F(0.0f, v, 0x142, 0xA);  // row_bcast:15
+    v += SVPF_DPP_F(0.0f, v, 0x143, 0xC);  // row_bcast:31
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 SVPF_DEV float wave_max(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, kWave));
-    return v;
+    const float lo = -3.402823466e38f;
+    v = fmaxf(v, SVPF_DPP_F(lo, v, 0x111, 0xF));
+    v = fmaxf(v, SVPF_DPP_F(lo, v, 0x112, 0xF));
+    v = fmaxf(v, SVPF_DPP_F(lo, v, 0x114, 0xF));
+    v = fmaxf(v, SVPF_DPP_F(lo, v, 0x118, 0xF));
+    v = fmaxf(v, SVPF_DPP_F(lo, v, 0x142, 0xA));
+    v = fmaxf(v, SVPF_DPP_F(lo, v, 0x143, 0xC));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 SVPF_DEV float wave_min(float v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
-    return v;
+    const float hi = 3.402823466e38f;
+    v = fminf(v, SVPF_DPP_F(hi, v, 0x111, 0xF));
+    v = fminf(v, SVPF_DPP_F(hi, v, 0x112, 0xF));
+    v = fminf(v, SVPF_DPP_F(hi, v, 0x114, 0xF));
+    v = fminf(v, SVPF_DPP_F(hi, v, 0x118, 0xF));
+    v = fminf(v, SVPF_DPP_F(hi, v, 0x142, 0xA));
+    v = fminf(v, SVPF_DPP_F(hi, v, 0x143, 0xC));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
 }
 SVPF_DEV uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-    return v;
+    v += SVPF_DPP_U(0u, v, 0x111, 0xF);
+    v += SVPF_DPP_U(0u, v, 0x112, 0xF);
+    v += SVPF_DPP_U(0u, v, 0x114, 0xF);
+    v += SVPF_DPP_U(0u, v, 0x118, 0xF);
+    v += SVPF_DPP_U(0u, v, 0x142, 0xA);
+    v += SVPF_DPP_U(0u, v, 0x143, 0xC);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
-// Reduce across the CS adjacent lanes that share one particle row.
+// The value of lane (l ^ J) for J < 64: DPP for partners inside a 16-lane
+// row (quad_perm for 1 and 2, row_shl / row_shr for 4 and 8), ds_bpermute
+// across rows.  All lanes of the wave must be active.
+template <int J>
+SVPF_DEV float xor_lane(float x) {
+    const int lane = threadIdx.x & (kWave - 1);
+    if constexpr (J == 1) {
+        return SVPF_DPP_F(x, x, 0xB1, 0xF);  // quad_perm [1,0,3,2]
+    } else if constexpr (J == 2) {
+        return SVPF_DPP_F(x, x, 0x4E, 0xF);  // quad_perm [2,3,0,1]
+    } else if constexpr (J == 4) {
+        float up = SVPF_DPP_F(x, x, 0x104, 0xF);  // row_shl:4, lane l reads l + 4
+        float dn = SVPF_DPP_F(x, x, 0x114, 0xF);  // row_shr:4, lane l reads l - 4
+        return (lane & 4) ? dn : up;
+    } else if constexpr (J == 8) {
+        float up = SVPF_DPP_F(x, x, 0x108, 0xF);
+        float dn = SVPF_DPP_F(x, x, 0x118, 0xF);
+        return (lane & 8) ? dn : up;
+    } else {
+        return __shfl_xor(x, J, kWave);
+    }
+}
+
+// Reduce across the CS adjacent lanes that share one particle row (all lanes active).
 template <int CS>
 SVPF_DEV float group_sum(float v) {
-#pragma unroll
-    for (int o = CS / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if constexpr (CS >= 2) v += xor_lane<1>(v);
+    if constexpr (CS >= 4) v += xor_lane<2>(v);
+    if constexpr (CS >= 8) v += xor_lane<4>(v);
+    if constexpr (CS >= 16) v += xor_lane<8>(v);
+    if constexpr (CS >= 32) v += xor_lane<16>(v);
+    if constexpr (CS >= 64) v += xor_lane<32>(v);
     return v;
 }
 template <int CS>
 SVPF_DEV float group_min(float v) {
-#pragma unroll
-    for (int o = CS / 2; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, kWave));
+    if constexpr (CS >= 2) v = fminf(v, xor_lane<1>(v));
+    if constexpr (CS >= 4) v = fminf(v, xor_lane<2>(v));
+    if constexpr (CS >= 8) v = fminf(v, xor_lane<4>(v));
+    if constexpr (CS >= 16) v = fminf(v, xor_lane<8>(v));
+    if constexpr (CS >= 32) v = fminf(v, xor_lane<16>(v));
+    if constexpr (CS >= 64) v = fminf(v, xor_lane<32>(v));
     return v;
 }
 
@@ -172,66 +246,389 @@ SVPF_DEV uint32_t block_sum_u32(uint32_t v, uint32_t* red) {
     return s;
 }
 
+// Inclusive prefix sum over the 64 lanes (the scan inside wave_sum_u32).
+SVPF_DEV uint32_t wave_scan_u32(uint32_t v) {
+    v += SVPF_DPP_U(0u, v, 0x111, 0xF);
+    v += SVPF_DPP_U(0u, v, 0x112, 0xF);
+    v += SVPF_DPP_U(0u, v, 0x114, 0xF);
+    v += SVPF_DPP_U(0u, v, 0x118, 0xF);
+    v += SVPF_DPP_U(0u, v, 0x142, 0xA);
+    v += SVPF_DPP_U(0u, v, 0x143, 0xC);
+    return v;
+}
+
 // ---------------------------------------------------------------------------
-// Bitonic sort of p2 floats in LDS (p2 a power of two), ascending.
+// Workgroup sort of n floats in LDS.
+//
+// Stage 1: every wave sorts runs of 128 values in registers (two per lane,
+// element lane and lane + 64) with a bitonic network; partners at lane
+// distance 1, 2 (quad_perm) and 4, 8 (row_shl / row_shr) come from DPP, 16
+// and 32 from ds_bpermute, 64 is the other register.  Stage 2: runs are
+// merged by rank -- an element's output index is its index in its own run
+// plus the number of smaller elements in the other runs of its merge group
+// (ties go to the lower run, so equal keys never collide) -- two runs at a
+// time until at most four remain, then all four at once.
 // ---------------------------------------------------------------------------
-template <int BS>
-SVPF_DEV void bitonic_sort(float* a, int p2) {
-    for (int k = 2; k <= p2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int t = threadIdx.x; t < (p2 >> 1); t += BS) {
-                int i = 2 * j * (t / j) + (t % j);  // lower index of the pair
-                int l = i + j;
-                bool up = ((i & k) == 0);
-                float x = a[i], y = a[l];
-                if ((x > y) == up) {
-                    a[i] = y;
-                    a[l] = x;
-                }
-            }
-            __syncthreads();
-        }
+template <int K, int J>
+SVPF_DEV void bitonic_step(float& x0, float& x1) {
+    if constexpr (J == 64) {
+        const float a = fminf(x0, x1), b = fmaxf(x0, x1);
+        x0 = a;
+        x1 = b;
+    } else {
+        const int lane = threadIdx.x & (kWave - 1);
+        const float y0 = xor_lane<J>(x0), y1 = xor_lane<J>(x1);
+        const bool lower = (lane & J) == 0;
+        const bool up0 = (lane & K) == 0;
+        const bool up1 = ((lane + 64) & K) == 0;
+        x0 = (lower == up0) ? fminf(x0, y0) : fmaxf(x0, y0);
+        x1 = (lower == up1) ? fminf(x1, y1) : fmaxf(x1, y1);
     }
 }
 
-// Number of j in (i, n) with fl(fl(s[j]-s[i])^2) <= x, for ascending s.  The
-// squared differences along a row are non-decreasing in j (rounding is
-// monotone), so a binary search finds the boundary.
-SVPF_DEV int row_count_le(const float* s, int i, int n, float x) {
-    int lo = i + 1, hi = n;  // first index with r2 > x lies in [lo, hi]
-    const float si = s[i];
+template <int K, int J>
+SVPF_DEV void bitonic_merge_steps(float& x0, float& x1) {
+    bitonic_step<K, J>(x0, x1);
+    if constexpr (J > 1) bitonic_merge_steps<K, J / 2>(x0, x1);
+}
+
+template <int K>
+SVPF_DEV void bitonic_stages(float& x0, float& x1) {
+    bitonic_merge_steps<K, K / 2>(x0, x1);
+    if constexpr (K < 128) bitonic_stages<K * 2>(x0, x1);
+}
+
+// Ascending sort of the 128 values {x0 of lane l = element l, x1 = element l + 64}.
+SVPF_DEV void wave_sort128(float& x0, float& x1) { bitonic_stages<2>(x0, x1); }
+
+// One merge level over full runs: groups of WAYS runs of length lr (a power
+// of two) from src[0, total) into dst, total a multiple of lr * WAYS and of
+// BS.  An element's output index is its index within its group's own run
+// plus the number of elements of every other run of the group that precede
+// it in a stable merge (<= x in lower runs, < x in higher runs), so equal
+// keys never collide.  Each thread ranks two elements at once; the binary
+// searches advance in lockstep, branch-free, and never leave their run, so
+// all loads of one step are in flight together.
+template <int BS, int WAYS>
+SVPF_DEV void merge_level(const float* src, float* dst, int total, int lr) {
+    const int gl = lr * WAYS;
+    const int lg = 31 - __clz(lr);
+    for (int e0 = threadIdx.x; e0 < total; e0 += 2 * BS) {
+        const bool two = e0 + BS < total;  // uniform: total is a multiple of BS
+        float x[2];
+        int out[2], own[2], pos[2][WAYS];
+        bool strict[2][WAYS];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int e = two ? e0 + k * BS : e0;
+            x[k] = src[e];
+            const int gs = e & ~(gl - 1);
+            const int q = (e - gs) >> lg;
+            out[k] = e - q * lr;  // group start + index within the own run
+            own[k] = q;
+#pragma unroll
+            for (int r = 0; r < WAYS; ++r) {
+                pos[k][r] = gs + r * lr - 1;  // last index known to precede x (the own run is searched too, unused)
+                strict[k][r] = r > q;
+            }
+        }
+        // steps lr/2 .. 1 reach at most the run's second-to-last element; its
+        // last element is tested after the loop
+        for (int step = lr >> 1; step > 0; step >>= 1) {
+            float u[2][WAYS];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int r = 0; r < WAYS; ++r) u[k][r] = src[pos[k][r] + step];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+#pragma unroll
+                for (int r = 0; r < WAYS; ++r) {
+                    const bool before = strict[k][r] ? (u[k][r] < x[k]) : (u[k][r] <= x[k]);
+                    pos[k][r] += step & -(int)before;
+                }
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int gs = (two ? e0 + k * BS : e0) & ~(gl - 1);
+#pragma unroll
+            for (int r = 0; r < WAYS; ++r) {
+                const int a = gs + r * lr;
+                const float u = src[a + lr - 1];
+                const bool before = strict[k][r] ? (u < x[k]) : (u <= x[k]);
+                const int rank = before ? lr : pos[k][r] + 1 - a;
+                out[k] += (r == own[k]) ? 0 : rank;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            if (k == 0 || two) dst[out[k]] = x[k];
+    }
+}
+
+// Sorts src[0, n) ascending into dst[0, n); src is overwritten.  Works over
+// the power-of-two domain ps = max(128, pow2_ceil(n)) with +inf padding, so
+// src and dst must hold ps floats when n > 128 (n floats otherwise); dst
+// ends with the padding in [n, ps).  Contains barriers: all threads of the
+// workgroup call it; src must be complete (synchronised) on entry, dst is
+// complete on return.
+template <int BS>
+SVPF_DEV void block_sort(float* src, float* dst, int n) {
+    constexpr int NW = BS / kWave;
+    constexpr float kInf = 3.402823466e38f;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    int ps = 128;
+    while (ps < n) ps <<= 1;
+    const int runs = ps / 128;
+    const int wlim = runs == 1 ? n : ps;  // one run: no padding written
+    int levels = 0;
+    for (int r = runs; r > 4; r >>= 1) ++levels;
+    if (runs > 1) ++levels;
+    float* a = (levels & 1) ? src : dst;  // stage 1 target, so the last level lands in dst
+    for (int c = w; c < runs; c += NW) {
+        const int i0 = c * 128 + lane, i1 = i0 + 64;
+        float x0 = i0 < n ? src[i0] : kInf;
+        float x1 = i1 < n ? src[i1] : kInf;
+        wave_sort128(x0, x1);
+        if (i0 < wlim) a[i0] = x0;
+        if (i1 < wlim) a[i1] = x1;
+    }
+    __syncthreads();
+    float* b = (a == src) ? dst : src;
+    int lr = 128, r = runs;
+    while (r > 4) {
+        merge_level<BS, 2>(a, b, ps, lr);
+        __syncthreads();
+        float* t = a;
+        a = b;
+        b = t;
+        lr *= 2;
+        r >>= 1;
+    }
+    if (r == 4) {
+        merge_level<BS, 4>(a, b, ps, lr);
+        __syncthreads();
+    } else if (r == 2) {
+        merge_level<BS, 2>(a, b, ps, lr);
+        __syncthreads();
+    }
+}
+
+// First index j in [lo, hi] with fl(fl(s[j]-si)^2) > x, for ascending s and a
+// row whose squared differences are non-decreasing in j (rounding is
+// monotone), given that the answer lies in [lo, hi].
+SVPF_DEV int row_first_gt(const float* s, float si, int lo, int hi, float x) {
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
         float d = s[mid] - si;
-        float r2 = d * d;
-        if (r2 <= x) lo = mid + 1;
+        if (d * d <= x) lo = mid + 1;
         else hi = mid;
     }
-    return lo - (i + 1);
+    return lo;
+}
+
+// The W lockstep searches of one row: res[q] = first j in [a, b] with
+// fl(fl(s[j]-si)^2) > xb[q].  Every j >= b fails the test (b ends the row's
+// bracket, and s[n] is +inf), so probes need no bracket check: they are
+// only clamped to n.
+template <int W>
+SVPF_DEV void row_search(const float* s, int n, float si, int a, int b, const uint32_t (&xb)[W], int (&res)[W]) {
+    int pos[W];
+#pragma unroll
+    for (int q = 0; q < W; ++q) pos[q] = a - 1;
+    const int len = b - a;
+    for (int step = len > 0 ? (1 << (31 - __clz(len))) : 0; step > 0; step >>= 1) {
+        float u[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) u[q] = s[min(pos[q] + step, n)];  // all W loads in flight
+#pragma unroll
+        for (int q = 0; q < W; ++q) {
+            const float d = u[q] - si;
+            pos[q] += step & -(int)(d * d <= __uint_as_float(xb[q]));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < W; ++q) res[q] = pos[q] + 1;
 }
 
 // Lower median (rank floor((M-1)/2), M = n(n-1)/2) of the pairwise squared
-// distances of the ascending array s[0, n).  Bisection over the bit patterns
-// of non-negative floats, which order like the unsigned integers: the result
-// is the bit pattern of an actual pair value, identical to what a sort of all
-// M float32 values selects.
-template <int BS>
-SVPF_DEV float median_pairwise_r2(const float* s, int n, uint32_t* red) {
+// distances of the ascending array s[0, n), bit-identical to sorting all M
+// float32 values.  s needs room for n + 1 floats: s[n] is set to +inf.
+//
+// Multi-way bisection over the bit patterns of non-negative floats (which
+// order like unsigned integers, and whose spacing is close to geometric):
+// each round counts, with W lockstep binary searches per row, the pairs at
+// or below W thresholds inside the bracket [L, H], keeps the sub-bracket
+// holding the rank, and narrows every row's index bracket [plo[i], phi[i])
+// to the pairs still inside.  The thresholds of the first round are spaced
+// 1/16 octave around `est` (the previous tick's value; when <= 0, half the
+// squared interquartile range, which the median of r^2 equals for normal
+// data); while the rank stays below (above) all of them the next round
+// extends the grid downwards (upwards) with steps 8x wider, and once it is
+// bracketed the bracket is split evenly.  Once at
+// most `cmax` candidate pairs remain they are gathered into `cand` and the
+// rank is read off by counting.  Scratch (LDS): plo, phi (n ints each), cand
+// (cmax floats), red (64 words).  The first KR row slots of each thread keep
+// their search results in registers; later slots search again.
+template <int BS, int KR>
+SVPF_DEV float median_pairwise_r2(float* s, int n, float est, int* plo, int* phi, float* cand, int cmax,
+                                  uint32_t* red) {
+    constexpr int W = 7;
+    constexpr int NW = BS / kWave;
+    constexpr uint32_t kSeedStep = 1u << 19;  // 1/16 octave of r^2 (in float bit patterns)
+    enum { SEED = 0, DOWN = 1, UP = 2, SPLIT = 3 };
+    constexpr int kScan = 48, kResult = 48 + NW;
+    static_assert(kResult < 64, "reduction scratch");
     if (n < 2) return 0.0f;
-    const uint64_t M = (uint64_t)n * (uint64_t)(n - 1) / 2;
-    const uint32_t need = (uint32_t)((M - 1) / 2) + 1u;  // count(<= x) must reach rank + 1
-    float dmax = s[n - 1] - s[0];
-    uint32_t lo = 0u, hi = __float_as_uint(dmax * dmax);
-    while (lo < hi) {
-        uint32_t mid = lo + ((hi - lo) >> 1);
-        float x = __uint_as_float(mid);
-        uint32_t c = 0;
-        for (int i = threadIdx.x; i < n - 1; i += BS) c += (uint32_t)row_count_le(s, i, n, x);
-        c = block_sum_u32<BS>(c, red);
-        if (c >= need) hi = mid;
-        else lo = mid + 1u;
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int rows = n - 1;
+    const uint32_t M = (uint32_t)(((uint64_t)n * (uint64_t)(n - 1)) / 2);
+    const uint32_t need = (M - 1u) / 2u + 1u;  // count(<= x) must reach rank + 1
+    const float dmax = s[n - 1] - s[0];
+    uint32_t L = 0u, H = __float_as_uint(dmax * dmax);
+    uint32_t cL = 0u, cH = M;  // pairs below L, pairs at or below H
+    for (int i = threadIdx.x; i < rows; i += BS) {
+        plo[i] = i + 1;
+        phi[i] = n;
     }
-    return __uint_as_float(hi);
+    if (threadIdx.x < 16) red[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) s[n] = 3.402823466e38f;
+    __syncthreads();
+    if (!(est > 0.0f && est < 3.0e38f)) {
+        const float iq = s[(3 * n) / 4] - s[n / 4];
+        est = 0.5f * iq * iq;
+    }
+    const uint32_t eb = (est > 0.0f && est < 3.0e38f) ? __float_as_uint(est) : 0u;
+    int mode = (eb > L && eb < H) ? SEED : SPLIT;
+    int64_t anchor = (int64_t)eb, step = (int64_t)kSeedStep;
+    int round = 0;
+    while (cH - cL > (uint32_t)cmax && L < H) {
+        // thresholds, non-decreasing in q, inside [L, H)
+        auto thr = [&](int q) -> uint32_t {
+            int64_t x;
+            if (mode == SEED) x = anchor + (int64_t)(q - W / 2) * step;
+            else if (mode == DOWN) x = anchor - (int64_t)(W - q) * step;
+            else if (mode == UP) x = anchor + (int64_t)(q + 1) * step;
+            else return L + (uint32_t)(((uint64_t)(H - L) * (uint64_t)(q + 1)) / (W + 1));
+            x = x < (int64_t)L ? (int64_t)L : x;
+            x = x > (int64_t)H - 1 ? (int64_t)H - 1 : x;
+            return (uint32_t)x;
+        };
+        uint32_t xb[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) xb[q] = thr(q);
+        uint32_t c[W];
+#pragma unroll
+        for (int q = 0; q < W; ++q) c[q] = 0u;
+        int keep[KR][W];
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int i = threadIdx.x + k * BS;
+            if (i < rows) {
+                row_search<W>(s, n, s[i], plo[i], phi[i], xb, keep[k]);
+#pragma unroll
+                for (int q = 0; q < W; ++q) c[q] += (uint32_t)(keep[k][q] - (i + 1));
+            }
+        }
+        for (int i = threadIdx.x + KR * BS; i < rows; i += BS) {
+            int res[W];
+            row_search<W>(s, n, s[i], plo[i], phi[i], xb, res);
+#pragma unroll
+            for (int q = 0; q < W; ++q) c[q] += (uint32_t)(res[q] - (i + 1));
+        }
+        // counts: three rotating 16-word buffers, one barrier per round
+        uint32_t* buf = red + 16 * (round % 3);
+#pragma unroll
+        for (int q = 0; q < W; ++q) c[q] = wave_sum_u32(c[q]);
+        if (lane == 0) {
+#pragma unroll
+            for (int q = 0; q < W; ++q) atomicAdd(&buf[q], c[q]);
+        }
+        if (threadIdx.x < 16) red[16 * ((round + 1) % 3) + threadIdx.x] = 0u;
+        __syncthreads();
+        int qs = W;  // first threshold reaching the rank (W: none, the upper bracket end)
+#pragma unroll
+        for (int q = W - 1; q >= 0; --q)
+            if (buf[q] >= need) qs = q;
+        qs = __builtin_amdgcn_readfirstlane(qs);
+        const uint32_t xlo = qs > 0 ? thr(qs - 1) : 0u, xhi = qs < W ? thr(qs) : 0u;
+        const uint32_t clo = qs > 0 ? buf[qs - 1] : 0u, chi = qs < W ? buf[qs] : 0u;
+        if ((mode == SEED || mode == DOWN) && qs == 0) {
+            anchor = (int64_t)thr(0);  // still below the grid: extend it downwards
+            step *= 8;
+            mode = DOWN;
+        } else if ((mode == SEED || mode == UP) && qs == W) {
+            anchor = (int64_t)thr(W - 1);
+            step *= 8;
+            mode = UP;
+        } else {
+            mode = SPLIT;
+        }
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            const int i = threadIdx.x + k * BS;
+            if (i < rows) {
+                int na = plo[i], nb = phi[i];
+#pragma unroll
+                for (int q = 0; q < W; ++q) {  // bit blends, not selects: the results stay in registers
+                    const int ma = -(int)(q == qs - 1), mb = -(int)(q == qs);
+                    na = (keep[k][q] & ma) | (na & ~ma);
+                    nb = (keep[k][q] & mb) | (nb & ~mb);
+                }
+                plo[i] = na;
+                phi[i] = nb;
+            }
+        }
+        for (int i = threadIdx.x + KR * BS; i < rows; i += BS) {
+            const float si = s[i];
+            int na = plo[i], nb = phi[i];
+            if (qs > 0) na = row_first_gt(s, si, na, nb, __uint_as_float(xlo));
+            if (qs < W) nb = row_first_gt(s, si, na, nb, __uint_as_float(xhi));
+            plo[i] = na;
+            phi[i] = nb;
+        }
+        if (qs > 0) {
+            L = xlo + 1u;
+            cL = clo;
+        }
+        if (qs < W) {
+            H = xhi;
+            cH = chi;
+        }
+        ++round;
+    }
+    if (L >= H) return __uint_as_float(H);
+    // gather the cH - cL candidates (pairs with r^2 in [L, H]) and select by rank
+    uint32_t cnt = 0u;
+    for (int i = threadIdx.x; i < rows; i += BS) cnt += (uint32_t)(phi[i] - plo[i]);
+    const uint32_t incl = wave_scan_u32(cnt);
+    if (lane == kWave - 1) red[kScan + w] = incl;
+    __syncthreads();
+    uint32_t off = incl - cnt;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) off += (q < w) ? red[kScan + q] : 0u;
+    for (int i = threadIdx.x; i < rows; i += BS) {
+        const float si = s[i];
+        for (int j = plo[i]; j < phi[i]; ++j) {
+            const float d = s[j] - si;
+            cand[off++] = d * d;
+        }
+    }
+    __syncthreads();
+    const int C = (int)(cH - cL);
+    const uint32_t r0 = need - cL - 1u;
+    for (int k = threadIdx.x; k < C; k += BS) {
+        const float v = cand[k];
+        uint32_t below = 0u, upto = 0u;
+        for (int m = 0; m < C; ++m) {
+            const float u = cand[m];
+            below += (u < v) ? 1u : 0u;
+            upto += (u <= v) ? 1u : 0u;
+        }
+        if (below <= r0 && r0 < upto) red[kResult] = __float_as_uint(v);
+    }
+    __syncthreads();
+    return __uint_as_float(red[kResult]);
 }
 
 SVPF_DEV float clamp_logvol(float h) { return fminf(fmaxf(h, -15.0f), 5.0f); }

@@ -33,7 +33,9 @@ struct alignas(16) TickScalars {
     float ksd;           // KSD of the latest evaluated iteration
     int stop;            // early-stop decision broadcast
     int budget;
-    int pad[3];
+    float rs;            // 1 / bw
+    float med_est;       // previous tick's median pairwise r^2 (selection seed; 0 = none)
+    int pad;
 };
 
 // LDS layout for a workgroup running filters of up to nmax particles.
@@ -44,8 +46,11 @@ struct LdsLayout {
         while (p < n) p <<= 1;
         return p;
     }
+    // the means / sort buffer: a power of two above n (the median's sorted
+    // array is followed by a +inf sentinel)
+    __host__ __device__ static int mbuf_len(int n) { return pow2_ceil(n + 1 < 4 ? 4 : n + 1); }
     __host__ __device__ static size_t bytes(int nmax) {
-        int p2 = pow2_ceil(nmax < 4 ? 4 : nmax);
+        int p2 = mbuf_len(nmax);
         size_t b = (size_t)nmax * 16;          // hsc records
         b += (size_t)p2 * 4;                   // mixture means / sort buffer
         b += 64 * 4;                           // reduction scratch
@@ -78,9 +83,9 @@ struct Engine {
     float h[R], hp[R], v[R], lw[R], s[R];
 
     SVPF_DEV void bind(char* lds, int n_, int nmax) {
-        n = n_;
-        p2 = LdsLayout::pow2_ceil(n < 4 ? 4 : n);
-        int p2max = LdsLayout::pow2_ceil(nmax < 4 ? 4 : nmax);
+        n = __builtin_amdgcn_readfirstlane(n_);  // uniform: loop bounds live in SGPRs
+        p2 = LdsLayout::mbuf_len(n);
+        int p2max = LdsLayout::mbuf_len(nmax);
         hsc = reinterpret_cast<float4*>(lds);
         mbuf = reinterpret_cast<float*>(lds + (size_t)nmax * 16);
         red = mbuf + p2max;
@@ -122,44 +127,67 @@ struct Engine {
     // ---------------------------------------------------------------------
     // Mixture-prior gradient (src/svpf_opt_kernels.cu:354-521), Gaussian state
     // prior: g_prior(h_j) = sum_i r_i (m_i - h_j)/sigma^2 / (sum_i r_i + 1e-8),
-    // r_i = exp(-(h_j-m_i)^2/(2 sigma^2) - max), evaluated as
-    // exp2((dmin - d^2) * log2(e)/(2 sigma^2)).
+    // r_i = exp(-(h_j-m_i)^2/(2 sigma^2) - max)  (:453-470).
+    //
+    // Evaluated in units scaled by sq = sqrt(log2(e) / (2 sigma^2)): with
+    // hh = sq h and mm = sq m (mbuf holds the mm, unsorted, padded with
+    // far-away sentinels to a multiple of 4), r_i = exp2(dd_min^2 - (hh - mm_i)^2).
+    // As in the reference, a first pass finds the shift dd_min^2 (the nearest
+    // component, the reference's max term) and a second pass accumulates:
+    // a pair costs sub + mul + min, then sub + fma + exp2 + two accumulates.
     // ---------------------------------------------------------------------
-    SVPF_DEV void prior_sums(float c2, float (&sr)[R], float (&srd)[R]) const {
-        float dmin[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) dmin[r] = 3.0e38f;
-        const int nq = p2 >> 2;  // mbuf is padded to p2 with far-away sentinels
-        const float4* m4 = reinterpret_cast<const float4*>(mbuf);
-        for (int q = cs; q < nq; q += CS) {
-            float4 m = m4[q];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                float d0 = h[r] - m.x, d1 = h[r] - m.y, d2 = h[r] - m.z, d3 = h[r] - m.w;
-                dmin[r] = fminf(fminf(dmin[r], d0 * d0), fminf(fminf(d1 * d1, d2 * d2), d3 * d3));
-            }
-        }
-        float off[R];
+    SVPF_DEV void prior_sums(float sq, float (&sr)[R], float (&srd)[R]) const {
+        float hh[R], off[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            dmin[r] = group_min<CS>(dmin[r]);
-            off[r] = dmin[r] * c2;
+            hh[r] = h[r] * sq;
+            off[r] = 3.0e38f;
             sr[r] = 0.f;
             srd[r] = 0.f;
         }
-        for (int q = cs; q < nq; q += CS) {
-            float4 m = m4[q];
+        const int nq = (n + 3) >> 2;
+        const float4* m4 = reinterpret_cast<const float4*>(mbuf);
+        // U loads in flight per step: at one wave per SIMD nothing else hides the LDS latency
+        constexpr int U = R >= 2 ? 2 : 4;
+        auto pass1 = [&](const float4& m) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float d0 = h[r] - m.x, d1 = h[r] - m.y, d2 = h[r] - m.z, d3 = h[r] - m.w;
-                float e0 = __builtin_amdgcn_exp2f(fmaf(-(d0 * d0), c2, off[r]));
-                float e1 = __builtin_amdgcn_exp2f(fmaf(-(d1 * d1), c2, off[r]));
-                float e2 = __builtin_amdgcn_exp2f(fmaf(-(d2 * d2), c2, off[r]));
-                float e3 = __builtin_amdgcn_exp2f(fmaf(-(d3 * d3), c2, off[r]));
+                float d0 = hh[r] - m.x, d1 = hh[r] - m.y, d2 = hh[r] - m.z, d3 = hh[r] - m.w;
+                off[r] = fminf(off[r], fminf(fminf(d0 * d0, d1 * d1), fminf(d2 * d2, d3 * d3)));
+            }
+        };
+        auto pass2 = [&](const float4& m) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                float d0 = hh[r] - m.x, d1 = hh[r] - m.y, d2 = hh[r] - m.z, d3 = hh[r] - m.w;
+                float e0 = __builtin_amdgcn_exp2f(fmaf(-d0, d0, off[r]));
+                float e1 = __builtin_amdgcn_exp2f(fmaf(-d1, d1, off[r]));
+                float e2 = __builtin_amdgcn_exp2f(fmaf(-d2, d2, off[r]));
+                float e3 = __builtin_amdgcn_exp2f(fmaf(-d3, d3, off[r]));
                 sr[r] += (e0 + e1) + (e2 + e3);
                 srd[r] = fmaf(e0, d0, fmaf(e1, d1, fmaf(e2, d2, fmaf(e3, d3, srd[r]))));
             }
+        };
+        int q = cs;
+        for (; q + (U - 1) * CS < nq; q += U * CS) {
+            float4 m[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) m[u] = m4[q + u * CS];
+#pragma unroll
+            for (int u = 0; u < U; ++u) pass1(m[u]);
         }
+        for (; q < nq; q += CS) pass1(m4[q]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) off[r] = group_min<CS>(off[r]);
+        q = cs;
+        for (; q + (U - 1) * CS < nq; q += U * CS) {
+            float4 m[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) m[u] = m4[q + u * CS];
+#pragma unroll
+            for (int u = 0; u < U; ++u) pass2(m[u]);
+        }
+        for (; q < nq; q += CS) pass2(m4[q]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             sr[r] = group_sum<CS>(sr[r]);
@@ -170,54 +198,66 @@ struct Engine {
     // ---------------------------------------------------------------------
     // IMQ Stein transport (src/svpf_opt_kernels.cu:527-599 plain,
     // :742-906 local Newton, :912-1103 full Newton) with the reference KSD
-    // (:618-709).  Per pair: d = h_i - h_j, t1 = d/ell^2, K = 1/(1 + d t1).
-    //   ks  += K s_j                    (attraction)
-    //   gk  += t1 K^2 c_j               (c_j = 2 sgn [/ curv_j for local Newton])
-    //   KSD: A2 += t1 K^2, A3 += s_j t1 K^2, A4 += K^2 (4 d t1 K - 1)
-    //   full Newton: hw += curv_j K + 2/ell^2 K^2 |3 d t1 - 1|, kn += K
+    // (:618-709), in positions scaled by rs = 1/bw: hsc[j] = {h_j rs, s_j,
+    // c_j rs, curv_j} with c_j = 2 sgn (local Newton: 2 sgn / curv_j).  Per
+    // pair, d = (h_i - h_j) rs, K = 1/(1 + d^2):
+    //   acc += K (s_j + c_j rs d K)          = K s_j + c_j (h_i - h_j)/ell^2 K^2
+    //   KSD: a1 += K s_j, a2 += d K^2, a3 += s_j d K^2, a4 += K^2 (4 d^2 K - 1)
+    //   full Newton: hw += curv_j K + 2/ell^2 K^2 |3 d^2 - 1|, kn += K
     // ---------------------------------------------------------------------
     template <int MODE, bool KSD>
-    SVPF_DEV void transport_sums(float ib, float (&ks)[R], float (&gk)[R], float (&a2)[R], float (&a3)[R],
-                                 float (&a4)[R], float (&hw)[R], float (&kn)[R]) const {
+    SVPF_DEV void transport_sums(float ib, float rs, float (&acc)[R], float (&a1)[R], float (&a2)[R],
+                                 float (&a3)[R], float (&a4)[R], float (&hw)[R], float (&kn)[R]) const {
+        float ht[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            ks[r] = gk[r] = a2[r] = a3[r] = a4[r] = hw[r] = kn[r] = 0.f;
+            ht[r] = h[r] * rs;
+            acc[r] = a1[r] = a2[r] = a3[r] = a4[r] = hw[r] = kn[r] = 0.f;
         }
         const float two_ib = 2.0f * ib;
-        for (int j = cs; j < n; j += CS) {
-            float4 e = hsc[j];
+        auto body = [&](const float4& e) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                float d = h[r] - e.x;
-                float t1 = d * ib;
-                float K = __builtin_amdgcn_rcpf(fmaf(d, t1, 1.0f));
-                float K2 = K * K;
-                ks[r] = fmaf(K, e.y, ks[r]);
-                float u = t1 * K2;
-                gk[r] = fmaf(u, e.z, gk[r]);
+                float d = ht[r] - e.x;
+                float K = __builtin_amdgcn_rcpf(fmaf(d, d, 1.0f));
+                float q = fmaf(e.z * d, K, e.y);
+                acc[r] = fmaf(K, q, acc[r]);
                 if (MODE == MODE_FULL) {
-                    float x = d * t1;
+                    float K2 = K * K;
                     hw[r] = fmaf(e.w, K, hw[r]);
-                    hw[r] = fmaf(two_ib * K2, fabsf(fmaf(3.0f, x, -1.0f)), hw[r]);
+                    hw[r] = fmaf(two_ib * K2, fabsf(fmaf(3.0f * d, d, -1.0f)), hw[r]);
                     kn[r] += K;
                 }
                 if (KSD) {
-                    float x = d * t1;
+                    float K2 = K * K;
+                    float u = d * K2;
+                    a1[r] = fmaf(K, e.y, a1[r]);
                     a2[r] += u;
                     a3[r] = fmaf(e.y, u, a3[r]);
-                    a4[r] = fmaf(K2, fmaf(4.0f, x * K, -1.0f), a4[r]);
+                    a4[r] = fmaf(K2, fmaf(4.0f * d * d, K, -1.0f), a4[r]);
                 }
             }
+        };
+        // U loads in flight per step: at one wave per SIMD nothing else hides the LDS latency
+        constexpr int U = R >= 4 ? 1 : 8 / R;
+        int j = cs;
+        for (; j + (U - 1) * CS < n; j += U * CS) {
+            float4 e[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) e[u] = hsc[j + u * CS];
+#pragma unroll
+            for (int u = 0; u < U; ++u) body(e[u]);
         }
+        for (; j < n; j += CS) body(hsc[j]);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            ks[r] = group_sum<CS>(ks[r]);
-            gk[r] = group_sum<CS>(gk[r]);
+            acc[r] = group_sum<CS>(acc[r]);
             if (MODE == MODE_FULL) {
                 hw[r] = group_sum<CS>(hw[r]);
                 kn[r] = group_sum<CS>(kn[r]);
             }
             if (KSD) {
+                a1[r] = group_sum<CS>(a1[r]);
                 a2[r] = group_sum<CS>(a2[r]);
                 a3[r] = group_sum<CS>(a3[r]);
                 a4[r] = group_sum<CS>(a4[r]);
@@ -291,6 +331,7 @@ struct Engine {
             sc->guide_mean = S.guide_mean;
             sc->guide_std = sqrtf(fmaxf(S.guide_var, 1e-6f));
             sc->budget = budget;
+            sc->med_est = S.med_r2;
             sc->stop = 0;
             sc->ksd = 0.f;
         }
@@ -380,18 +421,23 @@ struct Engine {
         // ---- bandwidth ----
         bandwidth(P, S, y_t);
         const float ib = sc->ib;
+        const float rs = sc->rs;
 
-        // ---- mixture-component means m_i from the previous tick's particles ----
+        // ---- mixture-component means m_i from the previous tick's particles,
+        //      scaled by sq ----
+        const float inv_s2 = 1.0f / (sigma * sigma);
+        const float sq = sqrtf(1.4426950408889634f * 0.5f * inv_s2);
         {
             const float lev_num = P.gamma * y_prev;
             if (cs == 0) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     int i = row(r);
-                    if (i < n) mbuf[i] = mu + P.rho * (hp[r] - mu) + lev_num / (expf(hp[r] * 0.5f) + 1e-8f);
+                    if (i < n) mbuf[i] = sq * (mu + P.rho * (hp[r] - mu) + lev_num / (expf(hp[r] * 0.5f) + 1e-8f));
                 }
             }
-            for (int i = n + threadIdx.x; i < p2; i += BS) mbuf[i] = 1.0e4f;  // sentinels: exp2 underflows to 0
+            for (int i = n + threadIdx.x; i < p2; i += BS) mbuf[i] = 1.0e18f;  // sentinels: exp2 underflows to 0
+            __syncthreads();
         }
 
         // ---- annealed Stein loop (src/svpf_optimized_graph.cu:1057-1317) ----
@@ -399,8 +445,6 @@ struct Engine {
         const int budget = sc->budget;
         const float base_step = P.base_step;
         const float temp = (F & SVPF_F_SVLD) ? P.temperature : 0.0f;
-        const float inv_s2 = 1.0f / (sigma * sigma);
-        const float c2 = 1.4426950408889634f * 0.5f * inv_s2;
         const float sgn2 = (F & SVPF_F_PAPER_SIGN) ? 2.0f : -2.0f;
         const int mode = (F & SVPF_F_NEWTON) ? ((F & SVPF_F_FULL_NEWTON) ? MODE_FULL : MODE_NEWTON) : MODE_PLAIN;
         const bool early = (F & SVPF_F_KSD_EARLY_STOP) != 0;
@@ -423,18 +467,18 @@ struct Engine {
                 const int it_row = total;
                 ++total;
                 __syncthreads();  // previous transport finished reading hsc; mbuf complete
-                gradient(P, beta, y_t, inv_s2, c2, sgn2, mode);
+                gradient(P, beta, y_t, inv_s2, sq, rs, sgn2, mode);
                 __syncthreads();  // hsc complete
                 float phi[R], ksdp[R];
                 if (mode == MODE_NEWTON) {
-                    if (want_ksd) phi_rows<MODE_NEWTON, true>(ib, inv_n, phi, ksdp);
-                    else phi_rows<MODE_NEWTON, false>(ib, inv_n, phi, ksdp);
+                    if (want_ksd) phi_rows<MODE_NEWTON, true>(ib, rs, inv_n, phi, ksdp);
+                    else phi_rows<MODE_NEWTON, false>(ib, rs, inv_n, phi, ksdp);
                 } else if (mode == MODE_FULL) {
-                    if (want_ksd) phi_rows<MODE_FULL, true>(ib, inv_n, phi, ksdp);
-                    else phi_rows<MODE_FULL, false>(ib, inv_n, phi, ksdp);
+                    if (want_ksd) phi_rows<MODE_FULL, true>(ib, rs, inv_n, phi, ksdp);
+                    else phi_rows<MODE_FULL, false>(ib, rs, inv_n, phi, ksdp);
                 } else {
-                    if (want_ksd) phi_rows<MODE_PLAIN, true>(ib, inv_n, phi, ksdp);
-                    else phi_rows<MODE_PLAIN, false>(ib, inv_n, phi, ksdp);
+                    if (want_ksd) phi_rows<MODE_PLAIN, true>(ib, rs, inv_n, phi, ksdp);
+                    else phi_rows<MODE_PLAIN, false>(ib, rs, inv_n, phi, ksdp);
                 }
                 // RMSProp + transport + Langevin noise (Jacobi: all phi from the iteration start)
 #pragma unroll
@@ -571,17 +615,19 @@ struct Engine {
     }
 
     // Gradient for own rows; writes the hsc records of own rows.
-    SVPF_DEV void gradient(const svpf_fparams& P, float beta, float y_t, float inv_s2, float c2, float sgn2, int mode) {
+    SVPF_DEV void gradient(const svpf_fparams& P, float beta, float y_t, float inv_s2, float sq, float rs, float sgn2,
+                           int mode) {
         float sr[R], srd[R];
-        prior_sums(c2, sr, srd);
+        prior_sums(sq, sr, srd);
         const uint32_t F = P.flags;
         const float nu = P.nu;
         const float eb = (F & SVPF_F_FAN) ? 1.0f : beta;
         const float ysq = y_t * y_t;
         const float log_y2 = logf(ysq + 1e-10f);
+        const float isq = inv_s2 / sq;  // srd is in sq-scaled units
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            float gp = (srd[r] * inv_s2) / (sr[r] + 1e-8f);
+            float gp = (srd[r] * isq) / (sr[r] + 1e-8f);
             gp = -gp;
             float hj = h[r];
             float vol = safe_expf(hj);
@@ -604,23 +650,23 @@ struct Engine {
             }
             s[r] = sj;
             int i = row(r);
-            if (i < n && cs == 0) hsc[i] = make_float4(hj, sj, cj, cv);
+            if (i < n && cs == 0) hsc[i] = make_float4(hj * rs, sj, cj * rs, cv);
         }
     }
 
     template <int MODE, bool KSD>
-    SVPF_DEV void phi_rows(float ib, float inv_n, float (&phi)[R], float (&ksdp)[R]) const {
-        float ks[R], gk[R], a2[R], a3[R], a4[R], hw[R], kn[R];
-        transport_sums<MODE, KSD>(ib, ks, gk, a2, a3, a4, hw, kn);
+    SVPF_DEV void phi_rows(float ib, float rs, float inv_n, float (&phi)[R], float (&ksdp)[R]) const {
+        float acc[R], a1[R], a2[R], a3[R], a4[R], hw[R], kn[R];
+        transport_sums<MODE, KSD>(ib, rs, acc, a1, a2, a3, a4, hw, kn);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            float p = (ks[r] + gk[r]) * inv_n;
+            float p = acc[r] * inv_n;
             if (MODE == MODE_FULL) {
                 float hb = fminf(fmaxf(hw[r] / fmaxf(kn[r], 1e-6f), 0.1f), 100.0f);
                 p = p / hb * 0.7f;
             }
             phi[r] = p;
-            if (KSD) ksdp[r] = s[r] * ks[r] + 2.0f * s[r] * a2[r] - 2.0f * a3[r] + 2.0f * ib * a4[r];
+            if (KSD) ksdp[r] = s[r] * a1[r] + 2.0f * rs * (s[r] * a2[r] - a3[r]) + 2.0f * ib * a4[r];
             else ksdp[r] = 0.f;
         }
     }
@@ -628,21 +674,27 @@ struct Engine {
     // Bandwidth: writes sc->ib, sc->bw, sc->bw_sq.  Uses mbuf as sort buffer.
     SVPF_DEV void bandwidth(const svpf_fparams& P, svpf_fstate& S, float y_t) {
         if (P.bandwidth_mode == SVPF_BW_MEDIAN) {
+            // the transport records are not live yet this tick: their LDS is the
+            // sort input, then the row brackets and the candidate buffer
+            float* tmp = reinterpret_cast<float*>(hsc);
             if (cs == 0) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     int i = row(r);
-                    if (i < n) mbuf[i] = h[r];
+                    if (i < n) tmp[i] = h[r];
                 }
             }
-            for (int i = n + threadIdx.x; i < p2; i += BS) mbuf[i] = 3.0e38f;
             __syncthreads();
-            bitonic_sort<BS>(mbuf, p2);
-            float med = median_pairwise_r2<BS>(mbuf, n, reinterpret_cast<uint32_t*>(red));
+            block_sort<BS>(tmp, mbuf, n);
+            int* plo = reinterpret_cast<int*>(hsc);
+            float med = median_pairwise_r2<BS, 2>(mbuf, n, sc->med_est, plo, plo + n, reinterpret_cast<float*>(plo + 2 * n),
+                                                  min(kMedianCands, 2 * n), reinterpret_cast<uint32_t*>(red));
             if (threadIdx.x == 0) {
+                S.med_r2 = med;
                 float bw = fminf(fmaxf(sqrtf(med), 0.01f), 2.0f);
                 sc->bw = bw;
                 sc->ib = 1.0f / (bw * bw);
+                sc->rs = 1.0f / bw;
                 sc->bw_sq = med;
             }
             __syncthreads();
@@ -696,6 +748,7 @@ struct Engine {
             }
             sc->bw = bw;
             sc->ib = 1.0f / (bw * bw);
+                sc->rs = 1.0f / bw;
         }
         __syncthreads();
     }

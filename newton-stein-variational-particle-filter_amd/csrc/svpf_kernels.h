@@ -35,3 +35,16 @@ hipError_t svpf_launch_batched(int bs, int tile, int grid, const svpf_fparams* p
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+// Cooperative latency kernel for one filter (256 threads per workgroup, CS lanes per row).
+int svpf_coop_rows(int cs);
+size_t svpf_coop_lds_bytes(int n);
+hipError_t svpf_launch_coop(int cs, int G, int stride, const svpf_fparams* params, svpf_fstate* fstate,
+                            const svpf_fdesc* desc, float* pool_h, float* pool_hp, float* pool_v, uint64_t* xbuf,
+                            int n, int t0, int t1, int cooperative, hipStream_t stream);
+#ifdef __cplusplus
+}
+#endif

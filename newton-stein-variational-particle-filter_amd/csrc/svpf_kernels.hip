@@ -166,3 +166,71 @@ extern "C" hipError_t svpf_launch_batched(int bs, int tile, int grid, const svpf
     }
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Cooperative latency kernel: one filter over G workgroups (see svpf_coop.hpp).
+// Workgroups whose index is not a multiple of `stride` exit at once, so with
+// stride 8 the G participants share one XCD (workgroups are dealt to the XCDs
+// round-robin) and exchange through one L2.
+// ---------------------------------------------------------------------------
+#include "svpf_coop.hpp"
+
+template <int BS, int CS>
+__global__ __launch_bounds__(BS) void svpf_coop_kernel(const svpf_fparams* __restrict__ params,
+                                                       svpf_fstate* __restrict__ fstate,
+                                                       const svpf_fdesc* __restrict__ desc, float* pool_h,
+                                                       float* pool_hp, float* pool_v, uint64_t* xbuf, int G,
+                                                       int stride, int t0, int t1) {
+    extern __shared__ __align__(16) char lds[];
+    if ((int)blockIdx.x % stride) return;
+    const int g = (int)blockIdx.x / stride;
+    if (g >= G) return;
+    const svpf_fdesc* D = desc;
+    const svpf_fparams* P = &params[D->param_idx];
+    CoopIO io{P, fstate, D, pool_h + D->off, pool_hp + D->off, pool_v + D->off, xbuf, G, g};
+    CoopEngine<BS, CS> e;
+    e.bind(lds, P->n, G, g);
+    if (!e.run(io, t0, t1) && threadIdx.x == 0) fstate->status = 2;
+}
+
+#define SVPF_COOP_CASE(CSV)                                                                                  \
+    case CSV:                                                                                                \
+        fn = (const void*)svpf_coop_kernel<256, CSV>;                                                        \
+        lds = CoopEngine<256, CSV>::lds_bytes(n);                                                            \
+        break;
+
+extern "C" int svpf_coop_rows(int cs) { return 256 / cs; }
+
+#ifdef SVPF_TRACE
+__device__ unsigned long long svpf_trace_buf[256 * 64];
+extern "C" hipError_t svpf_trace_dump(unsigned long long* out) {
+    return hipMemcpyFromSymbol(out, HIP_SYMBOL(svpf_trace_buf), sizeof(unsigned long long) * 256 * 64);
+}
+#endif
+
+extern "C" size_t svpf_coop_lds_bytes(int n) { return CoopEngine<256, 1>::lds_bytes(n); }
+
+extern "C" hipError_t svpf_launch_coop(int cs, int G, int stride, const svpf_fparams* params, svpf_fstate* fstate,
+                                       const svpf_fdesc* desc, float* pool_h, float* pool_hp, float* pool_v,
+                                       uint64_t* xbuf, int n, int t0, int t1, int cooperative, hipStream_t stream) {
+    const void* fn = nullptr;
+    size_t lds = 0;
+    switch (cs) {
+        SVPF_COOP_CASE(1)
+        SVPF_COOP_CASE(2)
+        SVPF_COOP_CASE(4)
+        SVPF_COOP_CASE(8)
+        SVPF_COOP_CASE(16)
+        SVPF_COOP_CASE(32)
+        default: return hipErrorInvalidValue;
+    }
+    if (G < 1 || stride < 1 || (long)G * (256 / cs) < n || G * 8 > n) return hipErrorInvalidValue;
+    if (lds > 159 * 1024) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    void* args[] = {(void*)&params, (void*)&fstate, (void*)&desc, (void*)&pool_h, (void*)&pool_hp, (void*)&pool_v,
+                    (void*)&xbuf, (void*)&G, (void*)&stride, (void*)&t0, (void*)&t1};
+    dim3 grid(G * stride), block(256);
+    if (cooperative) return hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream);
+    return hipLaunchKernel(fn, grid, block, args, lds, stream);
+}

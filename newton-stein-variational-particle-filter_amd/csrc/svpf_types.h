@@ -74,8 +74,10 @@ typedef struct svpf_fstate {
     float guide_var, bw_sq_ema, return_ema, return_var;
     float mu_state, mu_var, sigma_z_eff, y_prev;
     float last_loglik, last_vol, last_h_mean, last_bw;
-    float last_ksd, last_pred_loglik, last_guide_strength, pad;
-    float pad2[8];
+    float last_ksd, last_pred_loglik, last_guide_strength;
+    uint32_t epoch;            // last exchange epoch of the cooperative kernel
+    float med_r2;              // last median pairwise r^2 (seeds the next tick's selection)
+    float pad2[7];
 } svpf_fstate;
 
 // Per-tick output record (8 floats): written to out[t * SVPF_OUTW + k].

@@ -74,6 +74,7 @@ SVPF_ERR_UNSUPPORTED = 2
 SVPF_ERR_HIP = 3
 SVPF_ERR_NOT_INITIALIZED = 4
 SVPF_ERR_OUT_OF_MEMORY = 5
+SVPF_ERR_TIMEOUT = 6
 
 # Every symbol include/svpf.h declares (non-inline).
 EXPORTED = (
@@ -82,7 +83,8 @@ EXPORTED = (
     "svpf_graph_invalidate", "svpf_get_particles", "svpf_get_stats", "svpf_get_ess", "svpf_get_ksd_stats",
     "svpf_last_error", "svpf_set_particles", "svpf_get_rmsprop_state", "svpf_set_predrawn_noise",
     "svpf_set_scalar_state", "svpf_get_last_outputs", "svpf_batched_run", "svpf_tile_capacity_query",
-    "svpf_lds_bytes_query", "svpf_version",
+    "svpf_lds_bytes_query", "svpf_version", "svpf_set_coop_shape", "svpf_get_coop_shape",
+    "svpf_last_error_string",
 )
 
 _lib = None
@@ -131,6 +133,8 @@ def load(path: str | None = None):
     L.svpf_get_ksd_stats.argtypes = [SP, _FP, ctypes.POINTER(c_int)]
     L.svpf_last_error.restype = c_int
     L.svpf_last_error.argtypes = [SP]
+    L.svpf_last_error_string.restype = ctypes.c_char_p
+    L.svpf_last_error_string.argtypes = [SP]
     L.svpf_set_particles.restype = c_int
     L.svpf_set_particles.argtypes = [SP, c_void_p, c_void_p, c_void_p]
     L.svpf_get_rmsprop_state.restype = c_int
@@ -149,6 +153,10 @@ def load(path: str | None = None):
     L.svpf_tile_capacity_query.argtypes = [c_int, c_int]
     L.svpf_lds_bytes_query.restype = ctypes.c_size_t
     L.svpf_lds_bytes_query.argtypes = [c_int]
+    L.svpf_set_coop_shape.restype = c_int
+    L.svpf_set_coop_shape.argtypes = [SP, c_int, c_int]
+    L.svpf_get_coop_shape.restype = c_int
+    L.svpf_get_coop_shape.argtypes = [SP, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
     L.svpf_version.restype = ctypes.c_char_p
     L.svpf_version.argtypes = []
     if path is None:
@@ -157,8 +165,7 @@ def load(path: str | None = None):
 
 
 def check(state_ptr, what: str):
-    rc = load().svpf_last_error(state_ptr)
+    L = load()
+    rc = L.svpf_last_error(state_ptr)
     if rc != SVPF_OK:
-        names = {1: "invalid argument", 2: "unsupported feature", 3: "HIP runtime error",
-                 4: "filter not initialized", 5: "out of memory"}
-        raise RuntimeError(f"{what} failed: {names.get(rc, rc)}")
+        raise RuntimeError(f"{what} failed: {L.svpf_last_error_string(state_ptr).decode()} (status {rc})")

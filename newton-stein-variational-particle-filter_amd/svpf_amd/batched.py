@@ -34,9 +34,9 @@ FSTATE_DTYPE = np.dtype([("timestep", "<i4"), ("guide_initialized", "<i4"), ("st
                         + [(f, "<f4") for f in ("ksd_prev", "vol_prev", "h_mean_prev", "guide_mean", "guide_var",
                                                 "bw_sq_ema", "return_ema", "return_var", "mu_state", "mu_var",
                                                 "sigma_z_eff", "y_prev", "last_loglik", "last_vol", "last_h_mean",
-                                                "last_bw", "last_ksd", "last_pred_loglik", "last_guide_strength",
-                                                "pad")]
-                        + [("pad2", "<f4", (8,))])
+                                                "last_bw", "last_ksd", "last_pred_loglik", "last_guide_strength")]
+                        + [("epoch", "<u4"), ("med_r2", "<f4")]
+                        + [("pad2", "<f4", (7,))])
 FDESC_DTYPE = np.dtype([("off", "<i8")] + [(f, "<u8") for f in ("y", "out", "z_pred", "u_pred", "xi", "u_rej", "z_rej",
                                                                   "log_w", "h_pred", "seed")]
                        + [("xi_rows", "<i4"), ("param_idx", "<i4"), ("filter_id", "<u4"), ("tile", "<i4")])

@@ -22,7 +22,9 @@ def _f32(a):
 class Filter:
     """A single SVPF filter driven through libsvpf_amd.so."""
 
-    def __init__(self, cfg: dict, stream=None):
+    def __init__(self, cfg: dict, stream=None, path: str = "auto", coop_shape=None):
+        """path: "auto" (the library default: cooperative multi-workgroup kernel
+        for n >= 256), "coop" or "single" (one workgroup per filter)."""
         self.L = _lib.load()
         self.cfg = dict(cfg)
         n = int(cfg["n"])
@@ -34,6 +36,19 @@ class Filter:
         self.params = _lib.SVPFParams(cfg["rho"], cfg["sigma_z"], cfg["mu"], cfg["gamma"])
         self._keep = []
         self.apply_config(cfg)
+        if path == "single":
+            self.state.use_persistent_kernel = 0
+        elif path == "coop":
+            self.state.use_persistent_kernel = 1
+        if coop_shape is not None:
+            rc = self.L.svpf_set_coop_shape(self.sp, int(coop_shape[0]), int(coop_shape[1]))
+            if rc:
+                raise RuntimeError(f"svpf_set_coop_shape{tuple(coop_shape)} rejected ({rc})")
+
+    def coop_shape(self):
+        cs, g, st = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self.L.svpf_get_coop_shape(self.sp, cs, g, st)
+        return cs.value, g.value, st.value
 
     @property
     def state(self) -> _lib.SVPFState:

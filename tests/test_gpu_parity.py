@@ -35,9 +35,8 @@ def dev(torch, a):
     return None if a is None else torch.tensor(np.ascontiguousarray(a, dtype=np.float32), device="cuda")
 
 
-def run_gpu(torch, cfg, y, noise, h0, per_tick=True):
-    n = cfg["n"]
-    f = Filter(cfg)
+def run_gpu(torch, cfg, y, noise, h0, path="auto", coop_shape=None):
+    f = Filter(cfg, path=path, coop_shape=coop_shape)
     f.set_particles(h0)
     tz = {k: dev(torch, noise.get(k)) for k in ("z_pred", "xi", "u_pred", "u_rej", "z_rej")}
     f.set_noise(tz["z_pred"], tz["xi"], tz["u_pred"], tz["u_rej"], tz["z_rej"])
@@ -64,14 +63,15 @@ def compare(rec, out, tol_hmean=1e-4, tol_ll=1e-3, tol_vol=1e-4):
     assert np.array_equal(rec[:, 6].astype(int), out["steps"])
 
 
-def test_one_tick_from_identical_state(torch_cuda):
+@pytest.mark.parametrize("path", ["single", "coop"])
+def test_one_tick_from_identical_state(torch_cuda, path):
     n = 512
     cfg = config.plain_config(n=n)
     rng = np.random.default_rng(11)
     h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=5)
     noise = data.predraw_noise(1, n, 5, seed=6)
     y = np.array([0.42], dtype=np.float32)
-    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0, path=path)
     o, out = run_oracle(cfg, y, noise, h0)
     h_gpu = f.particles()
     assert np.max(np.abs(h_gpu - o.h)) <= 1e-4
@@ -82,12 +82,13 @@ def test_one_tick_from_identical_state(torch_cuda):
     del rng
 
 
-def test_plain_filter_trajectory_matches_golden(torch_cuda):
+@pytest.mark.parametrize("path", ["single", "coop"])
+def test_plain_filter_trajectory_matches_golden(torch_cuda, path):
     from golden.make_golden import plain_inputs
 
     g = np.load(os.path.join(HERE, "golden", "plain_n512_t1000.npz"))
     cfg, y, h_true, noise, h0 = plain_inputs()
-    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0, path=path)
     out = {k: g[k] for k in ("h_mean", "vol", "loglik", "steps")}
     compare(rec, out)
     assert np.sqrt(np.mean((f.particles() - g["h_final"]) ** 2)) <= 1e-4
@@ -112,8 +113,9 @@ FEATURE_CONFIGS = {
 }
 
 
+@pytest.mark.parametrize("path", ["single", "coop"])
 @pytest.mark.parametrize("name", sorted(FEATURE_CONFIGS))
-def test_feature_configs_short_run(torch_cuda, name):
+def test_feature_configs_short_run(torch_cuda, name, path):
     n, T = 300, 25
     cfg = config.make_config(n=n, mu=-1.0, rho=0.97, sigma_z=0.15, gamma=-0.2, **FEATURE_CONFIGS[name])
     y, _ = data.sv_series(T, mu=-1.0, rho=0.97, sigma_z=0.15, nu=5.0, seed=3)
@@ -121,19 +123,20 @@ def test_feature_configs_short_run(torch_cuda, name):
     rows = cfg["stein_max_steps"]
     noise = data.predraw_noise(T, n, rows, seed=4, mim=True, rejuv=True)
     h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=5)
-    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0, path=path)
     o, out = run_oracle(cfg, y, noise, h0)
     compare(rec, out, tol_hmean=1e-4, tol_ll=2e-3, tol_vol=2e-4)
 
 
+@pytest.mark.parametrize("path", ["single", "coop"])
 @pytest.mark.parametrize("n", [1, 2, 3, 65, 400, 2048, 4096])
-def test_particle_counts(torch_cuda, n):
+def test_particle_counts(torch_cuda, n, path):
     T = 6
     cfg = config.plain_config(n=n)
     y, _ = data.sv_series(T, seed=8)
     noise = data.predraw_noise(T, n, 5, seed=9)
     h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=10)
-    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0, path=path)
     o, out = run_oracle(cfg, y, noise, h0)
     compare(rec, out, tol_ll=1e-3)
     assert np.max(np.abs(f.particles() - o.h)) <= 2e-4
@@ -214,3 +217,35 @@ def test_graph_capture_replays(torch_cuda):
     f.L.svpf_graph_invalidate(f.sp)
     assert not f.L.svpf_graph_is_captured(f.sp)
     assert all(np.isfinite(x).all() for x in a)
+
+
+@pytest.mark.parametrize("shape", [(4, 8), (8, 8), (16, 8), (32, 8), (8, 1), (16, 1)])
+def test_coop_shapes(torch_cuda, shape):
+    """The cooperative kernel at several workgroup counts / placements (N=512)."""
+    n, T = 512, 8
+    cfg = config.plain_config(n=n)
+    y, _ = data.sv_series(T, seed=21)
+    noise = data.predraw_noise(T, n, 5, seed=22)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=23)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0, path="coop", coop_shape=shape)
+    assert f.coop_shape()[0] == shape[0] and f.coop_shape()[2] == shape[1]
+    o, out = run_oracle(cfg, y, noise, h0)
+    compare(rec, out)
+    assert np.max(np.abs(f.particles() - o.h)) <= 1e-4
+
+
+def test_coop_device_run_equals_per_tick(torch_cuda):
+    torch = torch_cuda
+    n, T = 512, 30
+    cfg = config.plain_config(n=n)
+    y, _ = data.sv_series(T, seed=24)
+    noise = data.predraw_noise(T, n, 5, seed=25)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=26)
+    f1, rec = run_gpu(torch, cfg, y, noise, h0, path="coop")
+    f2 = Filter(cfg, path="coop")
+    f2.set_particles(h0)
+    f2.set_noise(dev(torch, noise["z_pred"]), dev(torch, noise["xi"]))
+    ll = torch.zeros(T, device="cuda")
+    f2.run_device(dev(torch, y), ll)
+    assert np.array_equal(ll.cpu().numpy(), rec[:, 0].astype(np.float32))
+    assert np.array_equal(f1.particles(), f2.particles())
