@@ -4,7 +4,7 @@ Workload: one filter, N=512 particles, T=1000 ticks of the synthetic series
 h_t = mu + rho (h_{t-1} - mu) + sigma_z eps_t, y_t = exp(h_t/2) eta_t with
 mu=-1, rho=0.97, sigma_z=0.15, eta ~ Student-t(5), seed 0.  Filter: 5 Stein
 iterations, beta 0.3 -> 1.0 linear, IMQ with ell^2 = median pairwise r^2,
-local Newton, paper sign, RMSProp, SVLD at temperature 1.  Initial particles
+local Newton, legacy (reference default) sign, RMSProp, SVLD at temperature 1.  Initial particles
 (seed 2) and predict / Langevin noise (seed 1) are pre-drawn on the host.
 
 The inputs are not stored: tests regenerate them from the seeds with

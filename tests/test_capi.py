@@ -100,3 +100,18 @@ def test_record_layouts_match_c(tmp_path):
             if not f.startswith("pad"):
                 assert int(got[f"{cname}.{f}"][0]) == dt.fields[f][1], (cname, f)
     assert [int(x) for x in got["flags"]] == [batched.F_EXACT_GRAD, batched.F_KSD_EARLY_STOP, batched.F_PAPER_SIGN]
+
+
+def test_pysvpf_surface_matches_reference_module():
+    """Method and property names of pysvpf.SVPF (py-torch/svpf_python.cpp:483-654)."""
+    from svpf_amd import pysvpf
+
+    names = ["initialize", "step", "run", "reset", "set_adaptive_mu", "set_adaptive_sigma", "set_adaptive_guide",
+             "set_asymmetric_rho", "set_mim", "set_svld", "set_annealing", "set_newton", "set_guide", "set_params",
+             "set_adaptive_beta", "set_full_newton", "set_exact_gradient", "set_stein_steps",
+             "set_guided_innovation", "set_rejuvenation", "get_particles", "get_ess", "invalidate_graph",
+             "is_graph_captured", "nu", "lik_offset", "n_particles", "n_stein_steps"]
+    for k in names:
+        assert hasattr(pysvpf.SVPF, k), k
+    r = pysvpf.SVPFResult(1.0, 2.0, 0.0, 3.0, 4.0)
+    assert (r.log_lik, r.vol_mean, r.h_mean, r.mu_estimate) == (1.0, 2.0, 3.0, 4.0)
