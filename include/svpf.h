@@ -291,6 +291,22 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
                      float* d_pool_v, const int* d_order, const int* group_start, const int* group_tile,
                      const int* group_bs, const int* group_grid, const int* group_nmax, int n_groups, int* d_queues,
                      int t0, int t1, hipStream_t stream);
+/* ------------------------------------------------------------------------ */
+/* Differentiable form (no reference counterpart: the reference has no
+ * adjoint; its PyTorch path differentiates by autograd,
+ * py-torch/diff_svpf_torch.py:147-217).  B filters of n particles (2..2048)
+ * with the plain feature set and per-filter theta = (mu, rho, sigma_z, nu).
+ * config is a host pointer to one svpf_dconfig, d_desc a device array of B
+ * svpf_ddesc (csrc/svpf_types.h).  svpf_diff_forward writes the predictive
+ * log-likelihood of every tick, the median bandwidth of every tick and a
+ * checkpoint (x, v) every checkpoint_every ticks; svpf_diff_backward takes
+ * dL/d pred_ll and writes dL/d theta (and dL/d h0), recomputing each
+ * checkpoint segment into the tape.  Both return SVPF_OK or an SVPF_ERR_*
+ * code.  svpf_diff_tape_floats is the tape size per filter. */
+size_t svpf_diff_tape_floats(int n_particles, int n_stein, int checkpoint_every);
+int svpf_diff_forward(const void* config, const void* d_desc, int B, hipStream_t stream);
+int svpf_diff_backward(const void* config, const void* d_desc, int B, hipStream_t stream);
+
 /* Largest particle count a tile of the given block size holds (0: unknown tile). */
 int svpf_tile_capacity_query(int tile, int block_size);
 /* Dynamic LDS bytes a workgroup needs for filters of up to nmax particles. */

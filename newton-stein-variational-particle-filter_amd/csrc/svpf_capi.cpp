@@ -1009,6 +1009,27 @@ int svpf_get_coop_shape(const SVPFState* s, int* lanes_per_row, int* workgroups,
     return SVPF_OK;
 }
 
+size_t svpf_diff_tape_floats(int n_particles, int n_stein, int checkpoint_every) {
+    if (n_particles < 1 || n_stein < 1 || checkpoint_every < 1) return 0;
+    return svpf_diff_slot_floats_dev(n_particles, n_stein) * (size_t)checkpoint_every;
+}
+
+int svpf_diff_forward(const void* config, const void* d_desc, int B, hipStream_t stream) {
+    if (!config || !d_desc || B < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    const svpf_dconfig* c = static_cast<const svpf_dconfig*>(config);
+    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    hipError_t e = svpf_launch_diff(0, c, static_cast<const svpf_ddesc*>(d_desc), B, stream);
+    return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+}
+
+int svpf_diff_backward(const void* config, const void* d_desc, int B, hipStream_t stream) {
+    if (!config || !d_desc || B < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    const svpf_dconfig* c = static_cast<const svpf_dconfig*>(config);
+    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    hipError_t e = svpf_launch_diff(1, c, static_cast<const svpf_ddesc*>(d_desc), B, stream);
+    return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+}
+
 int svpf_tile_capacity_query(int tile, int block_size) { return svpf_tile_capacity(tile, block_size); }
 size_t svpf_lds_bytes_query(int nmax) { return svpf_lds_bytes(nmax); }
 const char* svpf_version(void) { return "svpf-mi355x 0.1.0 (gfx950)"; }

