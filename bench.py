@@ -12,7 +12,9 @@ same GPU (per-tick C-ABI step of single filters, measured on a stratified
 subset and weighted to the workload's N mix) and the ratio; the single-filter
 N=512 device-loop microseconds per step and the interactive per-tick latency;
 the roofline of the dominant kernel (the N=2048 tile group) against the
-FP32 vector peak; the oracle timed on the host cores as CPU baseline.
+FP32 vector peak; the differentiable filter (B=256, N=1024, T=500): forward
+and forward + adjoint with checkpoints every 32 ticks and with the full tape;
+the dense PyTorch restatement timed on the host cores as CPU baseline.
 """
 import argparse
 import json
@@ -148,8 +150,43 @@ def time_single(torch, T=1000, warm=50):
     return us_device, float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
 
+def cpu_threads():
+    """Host threads for the CPU baselines: the box's CPU share (at most 16)."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def cpu_baseline(cfgs, ys, h0s, n, ticks=3):
-    """FP64 oracle, single thread, one filter per N bucket for `ticks` ticks."""
+    """The dense PyTorch restatement of the filter tick (float32, all host
+    threads): one filter per N bucket, one warm-up tick then `ticks` timed
+    ticks, weighted to the workload's N mix."""
+    import torch
+
+    from oracle import torch_svpf
+
+    torch.set_num_threads(cpu_threads())
+    per_bucket = {}
+    rng = np.random.default_rng(7)
+    for N in sorted(set(n.tolist())):
+        b = int(np.nonzero(n == N)[0][0])
+        c = cfgs[b]
+        theta = [torch.tensor(c[k], dtype=torch.float32) for k in ("mu", "rho", "sigma_z", "nu")]
+        x = torch.tensor(h0s[b], dtype=torch.float32)
+        v = torch.zeros_like(x)
+        z = torch.tensor(rng.standard_normal((ticks + 1, N)), dtype=torch.float32)
+        xi = torch.tensor(rng.standard_normal((ticks + 1, S_ITERS, N)), dtype=torch.float32)
+        y = torch.tensor(ys[b], dtype=torch.float32)
+        with torch.no_grad():
+            x, v, _ = torch_svpf.tick(x, v, y[0], torch.tensor(0.0), z[0], xi[0], theta, c)
+            t0 = time.perf_counter()
+            for t in range(1, ticks + 1):
+                x, v, _ = torch_svpf.tick(x, v, y[t], y[t - 1], z[t], xi[t], theta, c)
+            per_bucket[N] = (time.perf_counter() - t0) / ticks
+    mix = np.mean([per_bucket[int(N)] for N in n])
+    return 1.0 / mix, per_bucket
+
+
+def oracle_fp64_rate(cfgs, ys, h0s, n, ticks=2):
+    """Secondary reference: the scalar FP64 C oracle on one thread."""
     import oracle
 
     per_bucket = {}
@@ -163,7 +200,75 @@ def cpu_baseline(cfgs, ys, h0s, n, ticks=3):
             yp = float(ys[b, t])
         per_bucket[N] = (time.perf_counter() - t0) / ticks
     mix = np.mean([per_bucket[int(N)] for N in n])
-    return 1.0 / mix, per_bucket
+    return 1.0 / mix
+
+
+def diff_workload(B=256, N=1024, T=500):
+    """Differentiable workload: B filters of N particles over T ticks with per-filter theta and series."""
+    from svpf_amd import data
+
+    _, mu, rho, sz, nu = data.batched_thetas(B, seed=3)
+    theta = np.stack([mu, rho, sz, nu], 1).astype(np.float32)
+    ys = np.zeros((B, T), np.float32)
+    h0 = np.zeros((B, N), np.float32)
+    for b in range(B):
+        ys[b], _ = data.sv_series(T, mu=mu[b], rho=rho[b], sigma_z=sz[b], nu=nu[b], seed=1000 + b)
+        h0[b] = data.initial_particles(N, mu[b], rho[b], sz[b], seed=2000 + b)
+    return theta, ys, h0
+
+
+def time_diff(torch, theta, ys, h0, checkpoint_every):
+    """Forward, and forward + adjoint, of the differentiable filter (HIP events on the launch stream)."""
+    from svpf_amd.diff import DiffBatch
+
+    bt = DiffBatch(ys, h0, n_stein=S_ITERS, checkpoint_every=checkpoint_every)
+    th = torch.tensor(theta, device="cuda")
+    g = -torch.ones((bt.B, bt.T), device="cuda")
+    bt.forward(th)
+    bt.backward(g)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(s)
+    bt.forward(th)
+    ev[1].record(s)
+    bt.backward(g)
+    ev[2].record(s)
+    torch.cuda.synchronize()
+    fwd = ev[0].elapsed_time(ev[1])
+    bwd = ev[1].elapsed_time(ev[2])
+    assert torch.isfinite(bt.gtheta).all()
+    tape_gb = bt.tape.numel() * 4 / 1e9
+    del bt
+    torch.cuda.empty_cache()
+    return fwd, bwd, tape_gb
+
+
+def cpu_diff_baseline(theta, ys, h0, ticks=2):
+    """Dense PyTorch restatement with autograd (float32, host threads): one filter, forward and forward+backward."""
+    import torch
+
+    from oracle import torch_svpf
+    from svpf_amd import config
+
+    torch.set_num_threads(cpu_threads())
+    N = h0.shape[1]
+    cfg = config.plain_config(n=N, n_stein=S_ITERS)
+    rng = np.random.default_rng(11)
+    z = torch.tensor(rng.standard_normal((ticks, N)), dtype=torch.float32)
+    xi = torch.tensor(rng.standard_normal((ticks, S_ITERS, N)), dtype=torch.float32)
+    y = torch.tensor(ys[0, :ticks], dtype=torch.float32)
+    h = torch.tensor(h0[0], dtype=torch.float32)
+    th = [torch.tensor(float(v), requires_grad=True) for v in theta[0]]
+    t0 = time.perf_counter()
+    with torch.no_grad():
+        torch_svpf.run(th, y, z, xi, h, cfg)
+    fwd = (time.perf_counter() - t0) / ticks
+    t0 = time.perf_counter()
+    pl, _, _ = torch_svpf.run(th, y, z, xi, h, cfg)
+    (-pl.sum()).backward()
+    fb = (time.perf_counter() - t0) / ticks
+    return fwd, fb
 
 
 def main():
@@ -212,10 +317,34 @@ def main():
             "single_n512_us_per_step_device_loop": us_dev,
             "single_n512_tick_latency_us_p50": p50,
             "single_n512_tick_latency_us_p99": p99,
-            "fwd_bwd_us_per_step": None,
         }
-    log("cpu baseline (oracle)")
-    cpu_rate, cpu_bucket = cpu_baseline(cfgs, ys, h0s, n)
+        log("differentiable filter B=256 N=1024 T=500")
+        theta, dys, dh0 = diff_workload()
+        T_d = dys.shape[1]
+        fwd32, bwd32, tape32 = time_diff(torch, theta, dys, dh0, 32)
+        fwdA, bwdA, tapeA = time_diff(torch, theta, dys, dh0, T_d)
+        log(f"diff: fwd {fwd32:.1f} ms, fwd+bwd {fwd32 + bwd32:.1f} ms (ckpt 32); "
+            f"fwd {fwdA:.1f} ms, fwd+bwd {fwdA + bwdA:.1f} ms (full tape)")
+        log("cpu baseline of the differentiable filter (dense torch + autograd)")
+        cfwd, cfb = cpu_diff_baseline(theta, dys, dh0)
+        diff_flops = float(dh0.shape[1]) ** 2 * 23 * S_ITERS * dh0.shape[0] * T_d
+        extras["diff"] = {
+            "workload": "B=256 filters, N=1024, T=500, S=5, per-filter theta and series, loss -sum_t pred_ll",
+            "fwd_ms": fwd32, "fwd_bwd_ms": fwd32 + bwd32, "fwd_us_per_step": fwd32 * 1e3 / T_d,
+            "fwd_bwd_us_per_step": (fwd32 + bwd32) * 1e3 / T_d, "fwd_bwd_over_fwd": (fwd32 + bwd32) / fwd32,
+            "checkpoint_every": 32, "tape_gb": tape32,
+            "full_tape": {"fwd_ms": fwdA, "fwd_bwd_ms": fwdA + bwdA, "fwd_bwd_over_fwd": (fwdA + bwdA) / fwdA,
+                          "tape_gb": tapeA},
+            "fwd_tflops": diff_flops / (fwd32 * 1e-3) / 1e12,
+            "cpu_baseline": {"fwd_s_per_filter_step": cfwd, "fwd_bwd_s_per_filter_step": cfb,
+                             "cores": cpu_threads(), "kind": "port",
+                             "sample": "dense torch float32 restatement, 1 filter N=1024, 2 ticks, autograd"},
+        }
+        extras["fwd_bwd_us_per_step"] = (fwd32 + bwd32) * 1e3 / T_d
+    log("cpu baseline (dense torch restatement)")
+    cpu_rate, cpu_bucket = cpu_baseline(cfgs, ys, h0s, n, ticks=8)
+    log("secondary reference (FP64 oracle, one thread)")
+    oracle_rate = oracle_fp64_rate(cfgs, ys, h0s, n)
 
     rec = {
         "metric": "filter-steps/s at B=4096 batched vs plain; us/step at N=512; fwd+bwd us/step",
@@ -238,6 +367,8 @@ def main():
         },
         "achieved_tflops_whole_batch": total_flops / wall / 1e12,
         "roofline": {
+            # the FP32 vector peak: the pair loops run on the VALU (no MFMA: a 1-D pairwise
+            # interaction is not a dense contraction); 157.3 TFLOP/s is also the FP32 matrix peak
             "bound": "valu",
             "achieved": achieved / 1e12,
             "peak": PEAK_FP32 / 1e12,
@@ -251,10 +382,12 @@ def main():
         "cpu_baseline": {
             "value": cpu_rate,
             "unit": "filter-steps/s",
-            "cores": 1,
+            "cores": cpu_threads(),
             "kind": "port",
-            "sample": "FP64 oracle, one filter per N bucket {128..2048} x 3 ticks, weighted to the workload N mix",
+            "sample": "dense PyTorch float32 restatement of the tick (oracle/torch_svpf.py), one filter per N "
+                      "bucket {128..2048} x 8 ticks after 1 warm-up, weighted to the workload N mix",
             "s_per_step_by_n": {str(k): v for k, v in cpu_bucket.items()},
+            "oracle_fp64_1thread_filter_steps_per_s": oracle_rate,
         },
     }
     rec.update(extras)

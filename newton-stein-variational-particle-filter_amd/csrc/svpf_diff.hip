@@ -25,6 +25,8 @@ __global__ __launch_bounds__(BS) void svpf_diff_forward_kernel(svpf_dconfig C, c
         e.v[r] = 0.f;
     }
     float med = 0.f, y_prev = 0.f;
+    const bool full_tape = C.C >= C.T;
+    const size_t slotf = diff_slot_floats(n, C.S);
     for (int t = 0; t < C.T; ++t) {
         if (t % C.C == 0) {
             float* ck = D.ckpt + (size_t)(t / C.C) * 2 * n;
@@ -37,7 +39,9 @@ __global__ __launch_bounds__(BS) void svpf_diff_forward_kernel(svpf_dconfig C, c
                 }
             }
         }
-        const float pl = e.template tick<false>(C, th, D, t, y_prev, nullptr, med);
+        // checkpoint interval >= T: the forward writes the whole tape and the backward skips the recompute
+        const float pl = full_tape ? e.template tick<true, true>(C, th, D, t, y_prev, D.tape + (size_t)t * slotf, med)
+                                   : e.template tick<false, true>(C, th, D, t, y_prev, nullptr, med);
         if (threadIdx.x == 0) D.pred_ll[t] = pl;
         y_prev = D.y[t];
     }
@@ -75,9 +79,11 @@ __global__ __launch_bounds__(BS) void svpf_diff_backward_kernel(svpf_dconfig C, 
             e.v[r] = i < n ? ck[n + i] : 0.f;
         }
         float med = 0.f, y_prev = t0 > 0 ? D.y[t0 - 1] : 0.f;
-        for (int t = t0; t < t1; ++t) {
-            e.template tick<true>(C, th, D, t, y_prev, D.tape + (size_t)(t - t0) * slotf, med);
-            y_prev = D.y[t];
+        if (C.C < T) {  // otherwise the forward wrote the tape
+            for (int t = t0; t < t1; ++t) {
+                e.template tick<true, false>(C, th, D, t, y_prev, D.tape + (size_t)(t - t0) * slotf, med);
+                y_prev = D.y[t];
+            }
         }
         for (int t = t1 - 1; t >= t0; --t) {
             const float yp = t > 0 ? D.y[t - 1] : 0.f;

@@ -24,10 +24,11 @@
 // iteration); then the ticks are reversed.  Per Stein iteration the adjoint
 // costs two O(N^2) passes:
 //   pass C (transport): one sweep over particle pairs accumulates the row sums
-//     sum_j s_j d K^2, sum_j c_j K^2 (1 - 4 d^2 K), sum_j s_j d^2 K^2,
-//     sum_j c_j d (K^2 + K^2 (1 - 4 d^2 K)) and, with the adjoint weights w_j,
-//     the column sums sum_j w_j K, sum_j w_j d K^2, sum_j w_j K^2 (1 - 4 d^2 K)
-//     (K symmetric, d antisymmetric, so a column sum is a row sum of w);
+//     sum_j s_j d K^2 and sum_j c_j K^2 (1 - 4 d^2 K) and, with the adjoint
+//     weights w_j, the column sums sum_j w_j K, sum_j w_j d K^2 and
+//     sum_j w_j K^2 (1 - 4 d^2 K) (K symmetric, d antisymmetric, so a column
+//     sum is a row sum of w); the bandwidth adjoint follows from these by
+//     translation invariance, with no further pair sums;
 //   pass B (mixture prior): every component i sums over particles j
 //     a_j rt_ij (1 - d_ij (d_ij - E_j d) / sigma^2), rt the normalised
 //     responsibilities rebuilt from the taped log-normaliser.
@@ -280,9 +281,9 @@ struct DiffEngine {
         }
     }
 
-    // ---- one tick forward.  RECORD: write the tape slot and take ell2 from
-    //      the forward's record instead of selecting it again. ----
-    template <bool RECORD>
+    // ---- one tick forward.  SELECT: select ell2 (the forward) or read the
+    //      forward's value (the recompute).  RECORD: write the tape slot. ----
+    template <bool RECORD, bool SELECT>
     SVPF_DEV float tick(const svpf_dconfig& C, const DiffTheta& th, const svpf_ddesc& D, int t, float y_prev,
                         float* slot, float& med_est) {
         const int S = C.S;
@@ -309,8 +310,14 @@ struct DiffEngine {
         const float pred_ll = logmeanexp(lw);
 
         float ell2;
-        if (RECORD) {
+        if (SELECT) {
+            ell2 = median_ell2(h, med_est);
+            med_est = ell2;
+            if (threadIdx.x == 0) D.ell2[t] = ell2;
+        } else {
             ell2 = D.ell2[t];
+        }
+        if (RECORD) {
             int a, b;
             const float bwr = sqrtf(ell2);
             if (bwr > 0.01f && bwr < 2.0f) {
@@ -324,10 +331,6 @@ struct DiffEngine {
                 sc[1] = __int_as_float(a);
                 sc[2] = __int_as_float(b);
             }
-        } else {
-            ell2 = median_ell2(h, med_est);
-            med_est = ell2;
-            if (threadIdx.x == 0) D.ell2[t] = ell2;
         }
         const float bw = fminf(fmaxf(sqrtf(ell2), 0.01f), 2.0f);
         const float rs = 1.0f / bw;
@@ -404,10 +407,10 @@ struct DiffEngine {
     }
 
     // ---- pass C: adjoint of the transport over all pairs ----
-    SVPF_DEV void pass_c(const float (&xs)[R], float (&R1)[R], float (&R2)[R], float (&R3)[R], float (&R4)[R],
-                         float (&C1)[R], float (&C2)[R], float (&C3)[R]) const {
+    SVPF_DEV void pass_c(const float (&xs)[R], float (&R1)[R], float (&R2)[R], float (&C1)[R], float (&C2)[R],
+                         float (&C3)[R]) const {
 #pragma unroll
-        for (int r = 0; r < R; ++r) R1[r] = R2[r] = R3[r] = R4[r] = C1[r] = C2[r] = C3[r] = 0.f;
+        for (int r = 0; r < R; ++r) R1[r] = R2[r] = C1[r] = C2[r] = C3[r] = 0.f;
         constexpr int U = R >= 2 ? 1 : 2;
         auto body = [&](const float4& e) {
 #pragma unroll
@@ -419,8 +422,6 @@ struct DiffEngine {
                 const float tt = fmaf(-4.0f * d * K, dK2, K2);  // K^2 (1 - 4 d^2 K)
                 R1[r] = fmaf(e.y, dK2, R1[r]);
                 R2[r] = fmaf(e.z, tt, R2[r]);
-                R3[r] = fmaf(e.y * d, dK2, R3[r]);
-                R4[r] = fmaf(e.z * d, tt + K2, R4[r]);
                 C1[r] = fmaf(e.w, K, C1[r]);
                 C2[r] = fmaf(e.w, dK2, C2[r]);
                 C3[r] = fmaf(e.w, tt, C3[r]);
@@ -551,16 +552,19 @@ struct DiffEngine {
             for (int r = 0; r < R; ++r)
                 if (valid(r)) rec[row(r)] = make_float4(xs[r], s[r], c[r] * rs, w[r]);
             __syncthreads();
-            float R1[R], R2[R], R3[R], R4[R], C1[R], C2[R], C3[R];
-            pass_c(xs, R1, R2, R3, R4, C1, C2, C3);
+            float R1[R], R2[R], C1[R], C2[R], C3[R];
+            pass_c(xs, R1, R2, C1, C2, C3);
             float a[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 a[r] = 0.f;
                 if (!valid(r)) continue;  // rows past n see nonzero column sums: keep them out of the adjoints
                 const float cr = c[r] * rs;
-                hb[r] += rs * (w[r] * (R2[r] - 2.0f * R1[r]) - 2.0f * s[r] * C2[r] - cr * C3[r]);
-                rsb += w[r] * (R4[r] - 2.0f * R3[r]) * bw;
+                // G = d(sum_i w_i phi_i)/d(h_i / bw); the transport sees positions only through
+                // differences, so the bandwidth adjoint is bw (sum_i x_i G_i - sum_i cr_i C2_i)
+                const float G = w[r] * (R2[r] - 2.0f * R1[r]) - 2.0f * s[r] * C2[r] - cr * C3[r];
+                hb[r] += rs * G;
+                rsb += (xs[r] * G - cr * C2[r]) * bw;
                 const float sbar = C1[r];
                 const float cbar = -C2[r] * rs;
                 const float gbar = gmask[r] * sbar * 0.7f / curv[r];
