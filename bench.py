@@ -164,6 +164,7 @@ def cpu_baseline(cfgs, ys, h0s, n, ticks=3):
     from oracle import torch_svpf
 
     torch.set_num_threads(cpu_threads())
+    ticks = max(1, min(ticks, ys.shape[1] - 1))
     per_bucket = {}
     rng = np.random.default_rng(7)
     for N in sorted(set(n.tolist())):
@@ -271,6 +272,25 @@ def cpu_diff_baseline(theta, ys, h0, ticks=2):
     return fwd, fb
 
 
+def profile_only(torch, what, B, K, W):
+    """One workload for a profiler run; prints what ran."""
+    if what == "dominant":
+        src = torch.ones(64 * 1024 * 1024, device="cuda")  # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        cfgs, ys, h0s, n = build_workload(B, W + K + 1)
+        ms, flops, nf, groups = time_dominant_kernel(torch, cfgs, ys, h0s, n, W, K)
+        log(f"copy 256 MiB; dominant group: {nf} filters, {K} ticks, {ms:.2f} ms, {flops / ms / 1e9:.2f} TFLOP/s")
+    elif what == "diff":
+        theta, dys, dh0 = diff_workload()
+        fwd, bwd, _ = time_diff(torch, theta, dys, dh0, 32)
+        log(f"diff: fwd {fwd:.1f} ms, bwd {bwd:.1f} ms")
+    else:
+        us, p50, p99 = time_single(torch, T=200, warm=20)
+        log(f"single N=512: {us:.1f} us/step device loop, tick p50 {p50:.1f} us")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +298,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--B", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
+    ap.add_argument("--only", choices=["dominant", "diff", "single"], default=None,
+                    help="run one workload once and exit (for profiler runs): the N=2048 tile group of the "
+                         "batched workload preceded by a 256 MiB device copy (traffic calibration), the "
+                         "differentiable filter, or the single N=512 filter")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: filters are independent and this build runs on one MI355X; "
@@ -288,9 +312,13 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     from svpf_amd import _lib
+    from svpf_amd.batched import TILES
 
     _lib.load()
     B, K, W = args.B, args.steps, args.warmup
+    if args.only:
+        profile_only(torch, args.only, B, K, W)
+        return
     log(f"building workload B={B} T={W + K}")
     cfgs, ys, h0s, n = build_workload(B, W + K + 1)
     log("batched run")
@@ -375,7 +403,8 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
             "traffic": None,
-            "kernel": f"svpf_batched_kernel<256,8,1> (N=2048 group, {nf_dom} filters, grid {int(groups['grid'][0])})",
+            "kernel": f"svpf_batched_kernel<{int(groups['bs'][0])},{TILES[int(groups['tile'][0])][0]},"
+                      f"{TILES[int(groups['tile'][0])][1]}> (N=2048 group, {nf_dom} filters, grid {int(groups['grid'][0])})",
             "kernel_ms": ms_dom,
             "flops_per_launch": flops_dom,
         },

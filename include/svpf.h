@@ -307,8 +307,13 @@ size_t svpf_diff_tape_floats(int n_particles, int n_stein, int checkpoint_every)
 int svpf_diff_forward(const void* config, const void* d_desc, int B, hipStream_t stream);
 int svpf_diff_backward(const void* config, const void* d_desc, int B, hipStream_t stream);
 
-/* Largest particle count a tile of the given block size holds (0: unknown tile). */
+/* Largest particle count a tile of the given block size holds (0: unknown tile).
+ * Block sizes 256 and 512 take tiles 0-7, 1024 takes tiles 0-4. */
 int svpf_tile_capacity_query(int tile, int block_size);
+/* Workgroups of that tile resident per CU for filters of up to nmax particles
+ * (HIP occupancy of the kernel with its LDS; 0: unknown tile).  Sizes the
+ * persistent grid of svpf_batched_run. */
+int svpf_tile_occupancy_query(int tile, int block_size, int nmax);
 /* Dynamic LDS bytes a workgroup needs for filters of up to nmax particles. */
 size_t svpf_lds_bytes_query(int nmax);
 /* Library version string. */

@@ -1031,6 +1031,7 @@ int svpf_diff_backward(const void* config, const void* d_desc, int B, hipStream_
 }
 
 int svpf_tile_capacity_query(int tile, int block_size) { return svpf_tile_capacity(tile, block_size); }
+int svpf_tile_occupancy_query(int tile, int block_size, int nmax) { return svpf_batched_wg_per_cu(block_size, tile, nmax); }
 size_t svpf_lds_bytes_query(int nmax) { return svpf_lds_bytes(nmax); }
 const char* svpf_version(void) { return "svpf-mi355x 0.1.0 (gfx950)"; }
 

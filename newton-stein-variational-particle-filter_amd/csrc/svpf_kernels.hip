@@ -15,7 +15,7 @@ using namespace svpf;
 namespace {
 
 template <int BS, int R, int CS>
-__device__ void run_filter(char* lds, int nmax, const svpf_fparams* params, svpf_fstate* fstate,
+__device__ __forceinline__ void run_filter(char* lds, int nmax, const svpf_fparams* params, svpf_fstate* fstate,
                            const svpf_fdesc* desc, int f, float* pool_h, float* pool_hp, float* pool_v, int t0,
                            int t1) {
     const svpf_fdesc* D = &desc[f];
@@ -86,6 +86,11 @@ extern "C" hipError_t svpf_kernels_init(void) {
     SVPF_SET_LDS_ATTR(512, 8, 1);
     SVPF_SET_LDS_ATTR(512, 2, 2);
     SVPF_SET_LDS_ATTR(512, 4, 2);
+    SVPF_SET_LDS_ATTR(1024, 1, 1);
+    SVPF_SET_LDS_ATTR(1024, 2, 1);
+    SVPF_SET_LDS_ATTR(1024, 4, 1);
+    SVPF_SET_LDS_ATTR(1024, 1, 2);
+    SVPF_SET_LDS_ATTR(1024, 1, 4);
     done = hipSuccess;
     return done;
 }
@@ -113,7 +118,56 @@ extern "C" hipError_t svpf_launch_init(float* h, float* hp, float* v, int n, flo
 
 extern "C" size_t svpf_lds_bytes(int nmax) { return LdsLayout::bytes(nmax); }
 
+#define SVPF_OCC_TILE(BSV, RV, CSV) fn = (const void*)svpf_batched_kernel<BSV, RV, CSV>
+
+// Workgroups of the given tile resident per CU for filters of up to nmax particles (0: unknown tile).
+extern "C" int svpf_batched_wg_per_cu(int bs, int tile, int nmax) {
+    if (svpf_kernels_init() != hipSuccess) return 0;
+    const void* fn = nullptr;
+    if (bs == 256) {
+        switch (tile) {
+            case SVPF_TILE_R1_C4: SVPF_OCC_TILE(256, 1, 4); break;
+            case SVPF_TILE_R1_C2: SVPF_OCC_TILE(256, 1, 2); break;
+            case SVPF_TILE_R1_C1: SVPF_OCC_TILE(256, 1, 1); break;
+            case SVPF_TILE_R2_C1: SVPF_OCC_TILE(256, 2, 1); break;
+            case SVPF_TILE_R4_C1: SVPF_OCC_TILE(256, 4, 1); break;
+            case SVPF_TILE_R8_C1: SVPF_OCC_TILE(256, 8, 1); break;
+            case SVPF_TILE_R2_C2: SVPF_OCC_TILE(256, 2, 2); break;
+            case SVPF_TILE_R4_C2: SVPF_OCC_TILE(256, 4, 2); break;
+            default: return 0;
+        }
+    } else if (bs == 512) {
+        switch (tile) {
+            case SVPF_TILE_R1_C4: SVPF_OCC_TILE(512, 1, 4); break;
+            case SVPF_TILE_R1_C2: SVPF_OCC_TILE(512, 1, 2); break;
+            case SVPF_TILE_R1_C1: SVPF_OCC_TILE(512, 1, 1); break;
+            case SVPF_TILE_R2_C1: SVPF_OCC_TILE(512, 2, 1); break;
+            case SVPF_TILE_R4_C1: SVPF_OCC_TILE(512, 4, 1); break;
+            case SVPF_TILE_R8_C1: SVPF_OCC_TILE(512, 8, 1); break;
+            case SVPF_TILE_R2_C2: SVPF_OCC_TILE(512, 2, 2); break;
+            case SVPF_TILE_R4_C2: SVPF_OCC_TILE(512, 4, 2); break;
+            default: return 0;
+        }
+    } else if (bs == 1024) {
+        switch (tile) {
+            case SVPF_TILE_R1_C4: SVPF_OCC_TILE(1024, 1, 4); break;
+            case SVPF_TILE_R1_C2: SVPF_OCC_TILE(1024, 1, 2); break;
+            case SVPF_TILE_R1_C1: SVPF_OCC_TILE(1024, 1, 1); break;
+            case SVPF_TILE_R2_C1: SVPF_OCC_TILE(1024, 2, 1); break;
+            case SVPF_TILE_R4_C1: SVPF_OCC_TILE(1024, 4, 1); break;
+            default: return 0;
+        }
+    } else {
+        return 0;
+    }
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, bs, LdsLayout::bytes(nmax)) != hipSuccess) return 0;
+    return nb;
+}
+
 extern "C" int svpf_tile_capacity(int tile, int bs) {
+    if (bs != 256 && bs != 512 && bs != 1024) return 0;
+    if (bs == 1024 && tile > SVPF_TILE_R4_C1) return 0;  // not instantiated at 1024 threads
     switch (tile) {
         case SVPF_TILE_R1_C4: return bs / 4;
         case SVPF_TILE_R1_C2: return bs / 2;
@@ -159,6 +213,15 @@ extern "C" hipError_t svpf_launch_batched(int bs, int tile, int grid, const svpf
             case SVPF_TILE_R8_C1: SVPF_LAUNCH_TILE(512, 8, 1); break;
             case SVPF_TILE_R2_C2: SVPF_LAUNCH_TILE(512, 2, 2); break;
             case SVPF_TILE_R4_C2: SVPF_LAUNCH_TILE(512, 4, 2); break;
+            default: return hipErrorInvalidValue;
+        }
+    } else if (bs == 1024) {
+        switch (tile) {
+            case SVPF_TILE_R1_C4: SVPF_LAUNCH_TILE(1024, 1, 4); break;
+            case SVPF_TILE_R1_C2: SVPF_LAUNCH_TILE(1024, 1, 2); break;
+            case SVPF_TILE_R1_C1: SVPF_LAUNCH_TILE(1024, 1, 1); break;
+            case SVPF_TILE_R2_C1: SVPF_LAUNCH_TILE(1024, 2, 1); break;
+            case SVPF_TILE_R4_C1: SVPF_LAUNCH_TILE(1024, 4, 1); break;
             default: return hipErrorInvalidValue;
         }
     } else {

@@ -85,6 +85,7 @@ EXPORTED = (
     "svpf_set_scalar_state", "svpf_get_last_outputs", "svpf_batched_run", "svpf_tile_capacity_query",
     "svpf_lds_bytes_query", "svpf_version", "svpf_set_coop_shape", "svpf_get_coop_shape",
     "svpf_last_error_string", "svpf_diff_tape_floats", "svpf_diff_forward", "svpf_diff_backward",
+    "svpf_tile_occupancy_query",
 )
 
 _lib = None
@@ -151,6 +152,8 @@ def load(path: str | None = None):
                                    c_void_p]
     L.svpf_tile_capacity_query.restype = c_int
     L.svpf_tile_capacity_query.argtypes = [c_int, c_int]
+    L.svpf_tile_occupancy_query.restype = c_int
+    L.svpf_tile_occupancy_query.argtypes = [c_int, c_int, c_int]
     L.svpf_lds_bytes_query.restype = ctypes.c_size_t
     L.svpf_lds_bytes_query.argtypes = [c_int]
     L.svpf_set_coop_shape.restype = c_int

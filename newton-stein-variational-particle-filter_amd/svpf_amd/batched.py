@@ -47,10 +47,12 @@ O_LOGLIK, O_VOL, O_HMEAN, O_KSD, O_BW, O_PRED_LL, O_STEPS, O_BWSQ = range(8)
 
 # Tile table (csrc/svpf_kernels.h): id -> (rows per lane, lanes per row)
 TILES = {0: (1, 4), 1: (1, 2), 2: (1, 1), 3: (2, 1), 4: (4, 1), 5: (8, 1), 6: (2, 2), 7: (4, 2)}
-# Resident workgroups per CU for each (block size, tile): from the kernels' VGPR use.
-_WG_PER_CU = {(256, 0): 4, (256, 1): 4, (256, 2): 4, (256, 3): 4, (256, 4): 3, (256, 5): 2, (256, 6): 3,
-              (256, 7): 3, (512, 0): 2, (512, 1): 2, (512, 2): 2, (512, 3): 1, (512, 4): 1, (512, 5): 1,
-              (512, 6): 1, (512, 7): 1}
+BLOCK_SIZES = (256, 512, 1024)
+
+
+def valid_tiles(bs):
+    """Tile ids instantiated at block size bs (1024 threads: rows <= 4, no split rows)."""
+    return [t for t in TILES if bs in (256, 512) or t <= 4]
 
 _TUNED_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")
 
@@ -221,7 +223,7 @@ class Batch:
         ggrid = []
         for g in range(len(keys)):
             nf = gstart[g + 1] - gstart[g]
-            per_cu = _WG_PER_CU.get((gbs[g], gtile[g]), 1)
+            per_cu = max(1, int(self.L.svpf_tile_occupancy_query(gtile[g], gbs[g], gnmax[g])))
             slots = cus * per_cu if persistent_grid is None else persistent_grid
             ggrid.append(max(1, min(nf, slots)))
         self.order = torch.tensor(order, dtype=torch.int32, device=device)

@@ -38,6 +38,8 @@ def test_tile_table():
     caps = [L.svpf_tile_capacity_query(t, 256) for t in range(8)]
     assert caps == [64, 128, 256, 512, 1024, 2048, 256, 512]
     assert L.svpf_tile_capacity_query(99, 256) == 0
+    assert [L.svpf_tile_capacity_query(t, 1024) for t in range(8)] == [256, 512, 1024, 2048, 4096, 0, 0, 0]
+    assert L.svpf_tile_capacity_query(2, 128) == 0
     assert L.svpf_lds_bytes_query(2048) >= 2048 * 20
 
 

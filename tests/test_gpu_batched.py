@@ -70,3 +70,38 @@ def test_batched_counter_generator_and_split_ticks(torch_cuda):
             hm.append(o.tick(float(y[b, t]), yp, seed=b, filter_id=b)["h_mean"])
             yp = float(y[b, t])
         assert np.sqrt(np.mean((oa[b, :, O_HMEAN] - np.array(hm)) ** 2)) <= 1e-4, b
+
+
+def every_tile():
+    from svpf_amd import batched
+
+    return [(bs, t) for bs in batched.BLOCK_SIZES for t in batched.valid_tiles(bs)]
+
+
+@pytest.mark.parametrize("bs,tile", every_tile())
+def test_every_tile_matches_oracle(torch_cuda, bs, tile):
+    """Each instantiated tile shape, at full and ragged occupancy, against the oracle."""
+    from svpf_amd import batched
+
+    cap = batched.tile_capacity(tile, bs)
+    full = min(cap, 2048)
+    sizes = sorted({full, max(2, cap // 2 + 3), max(2, cap // 4 + 1)})
+    sizes = [s for s in sizes if cap // 4 < s <= cap] or [full]
+    T = 3
+    cfgs, y, h0s = make_batch_inputs(len(sizes), T, sizes, seed=40 + tile)
+    noise = {"z_pred": [], "xi": []}
+    for b in range(len(sizes)):
+        nz = data.predraw_noise(T, cfgs[b]["n"], 5, seed=4000 + b)
+        noise["z_pred"].append(nz["z_pred"])
+        noise["xi"].append(nz["xi"])
+    bt = Batch(cfgs, y, h0s, noise=noise, tile_table={s: (bs, tile) for s in sizes})
+    assert set(zip(bt.groups["bs"].tolist(), bt.groups["tile"].tolist())) == {(bs, tile)}
+    bt.run(0, T)
+    torch_cuda.cuda.synchronize()
+    out = bt.outputs()
+    assert (bt.fstates()["status"] == 0).all()
+    for b in range(len(sizes)):
+        o = oracle.OracleFilter(cfgs[b], h0s[b])
+        ref = o.run(y[b], {"z_pred": noise["z_pred"][b], "xi": noise["xi"][b]})
+        assert np.sqrt(np.mean((out[b, :, O_HMEAN] - ref["h_mean"]) ** 2)) <= 1e-4, (b, sizes[b])
+        assert np.max(np.abs(bt.particles(b) - o.h)) <= 2e-4, (b, sizes[b])
