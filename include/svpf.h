@@ -307,6 +307,19 @@ size_t svpf_diff_tape_floats(int n_particles, int n_stein, int checkpoint_every)
 int svpf_diff_forward(const void* config, const void* d_desc, int B, hipStream_t stream);
 int svpf_diff_backward(const void* config, const void* d_desc, int B, hipStream_t stream);
 
+/* Two-factor form (no reference counterpart as a 2-D filter: the reference's
+ * two-factor branch, src/svpf_optimized_graph.cu:744-897, moves each factor
+ * with its own 1-D kernel).  B filters of n particles (2..2048) with
+ * x = (f, s), h = mu + f + s, a 2-D IMQ kernel and a 2x2 Newton
+ * preconditioner; theta = (mu, rho_f, sigma_f, rho_s, sigma_s, nu).  Records:
+ * svpf_d2config (host pointer) and svpf_d2desc (device array of B),
+ * csrc/svpf_types.h.  The forward writes per-tick records (predictive
+ * loglik, h / f / s means, vol, bandwidths) and checkpoints; the backward
+ * takes dL/d pred_ll and writes dL/d theta and dL/d x0. */
+size_t svpf_diff2d_tape_floats(int n_particles, int n_stein, int checkpoint_every);
+int svpf_diff2d_forward(const void* config, const void* d_desc, int B, hipStream_t stream);
+int svpf_diff2d_backward(const void* config, const void* d_desc, int B, hipStream_t stream);
+
 /* Largest particle count a tile of the given block size holds (0: unknown tile).
  * Block sizes 256 and 512 take tiles 0-7, 1024 takes tiles 0-4. */
 int svpf_tile_capacity_query(int tile, int block_size);

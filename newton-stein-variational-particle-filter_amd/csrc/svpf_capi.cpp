@@ -1030,6 +1030,27 @@ int svpf_diff_backward(const void* config, const void* d_desc, int B, hipStream_
     return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
 }
 
+size_t svpf_diff2d_tape_floats(int n_particles, int n_stein, int checkpoint_every) {
+    if (n_particles < 1 || n_stein < 1 || checkpoint_every < 1) return 0;
+    return svpf_diff2d_slot_floats_dev(n_particles, n_stein) * (size_t)checkpoint_every;
+}
+
+int svpf_diff2d_forward(const void* config, const void* d_desc, int B, hipStream_t stream) {
+    if (!config || !d_desc || B < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    const svpf_d2config* c = static_cast<const svpf_d2config*>(config);
+    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    hipError_t e = svpf_launch_diff2d(0, c, static_cast<const svpf_d2desc*>(d_desc), B, stream);
+    return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+}
+
+int svpf_diff2d_backward(const void* config, const void* d_desc, int B, hipStream_t stream) {
+    if (!config || !d_desc || B < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    const svpf_d2config* c = static_cast<const svpf_d2config*>(config);
+    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    hipError_t e = svpf_launch_diff2d(1, c, static_cast<const svpf_d2desc*>(d_desc), B, stream);
+    return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+}
+
 int svpf_tile_capacity_query(int tile, int block_size) { return svpf_tile_capacity(tile, block_size); }
 int svpf_tile_occupancy_query(int tile, int block_size, int nmax) { return svpf_batched_wg_per_cu(block_size, tile, nmax); }
 size_t svpf_lds_bytes_query(int nmax) { return svpf_lds_bytes(nmax); }

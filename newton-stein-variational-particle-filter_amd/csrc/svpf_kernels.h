@@ -60,3 +60,14 @@ hipError_t svpf_launch_diff(int backward, const svpf_dconfig* C, const svpf_ddes
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+// Two-factor form: tape floats per tick slot, forward (backward = 0) or adjoint launch.
+size_t svpf_diff2d_slot_floats_dev(int n, int S);
+hipError_t svpf_launch_diff2d(int backward, const svpf_d2config* C, const svpf_d2desc* desc, int B,
+                              hipStream_t stream);
+#ifdef __cplusplus
+}
+#endif

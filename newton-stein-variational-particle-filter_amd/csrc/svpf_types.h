@@ -148,6 +148,48 @@ typedef struct svpf_ddesc {
     int32_t pad;
 } svpf_ddesc;
 
+// ---------------------------------------------------------------------------
+// Two-factor (2-D) differentiable form: particle x = (f, s), log-vol
+// h = mu + f + s, per-filter theta = (mu, rho_f, sigma_f, rho_s, sigma_s, nu).
+// Arrays of particles are [2][n] (f slab, then s slab).
+// ---------------------------------------------------------------------------
+
+// Configuration shared by every filter of a launch (20 x 4 bytes).
+typedef struct svpf_d2config {
+    int32_t n;            // particles per filter
+    int32_t S;            // Stein iterations per tick
+    int32_t T;            // ticks
+    int32_t C;            // checkpoint interval (ticks)
+    int32_t paper_sign;   // 1: repulsive kernel gradient, 0: the reference default
+    int32_t pad0[3];
+    float lik_offset, temperature, rms_rho, rms_eps;
+    float pad1[8];
+} svpf_d2config;
+
+// Per-tick output record of the two-factor forward (8 floats).
+enum { SVPF_O2_PRED_LL = 0, SVPF_O2_HMEAN = 1, SVPF_O2_FMEAN = 2, SVPF_O2_SMEAN = 3, SVPF_O2_VOL = 4,
+       SVPF_O2_BWF = 5, SVPF_O2_BWS = 6, SVPF_O2W = 8 };
+
+// Per-filter descriptor (device pointers; optional ones may be null).
+typedef struct svpf_d2desc {
+    const float* theta;   // [6] mu, rho_f, sigma_f, rho_s, sigma_s, nu
+    const float* y;       // [T]
+    const float* x0;      // [2][n] initial factors (RMSProp state starts at 0)
+    const float* z;       // [T][2][n] pre-drawn predict normals, null: counter generator
+    const float* xi;      // [T][S][2][n] pre-drawn Langevin normals (with z)
+    float* out;           // [T][SVPF_O2W] per-tick records (forward output)
+    float* ell2;          // [T][2] median pairwise r^2 per coordinate (forward output, read by the backward)
+    float* ckpt;          // [ceil(T/C)][4][n] checkpoints (f, s, v_f, v_s)
+    float* tape;          // [C] tick slots of svpf_diff2d_slot_floats(n, S) floats
+    const float* gpl;     // [T] dL/d pred_ll (backward input)
+    float* gtheta;        // [6] dL/d theta (backward output)
+    float* gx0;           // [2][n] dL/d x0 (backward output, optional)
+    float* xT;            // [2][n] factors after the last tick (forward output, optional)
+    uint64_t seed;
+    uint32_t filter_id;
+    int32_t pad;
+} svpf_d2desc;
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,7 +85,7 @@ EXPORTED = (
     "svpf_set_scalar_state", "svpf_get_last_outputs", "svpf_batched_run", "svpf_tile_capacity_query",
     "svpf_lds_bytes_query", "svpf_version", "svpf_set_coop_shape", "svpf_get_coop_shape",
     "svpf_last_error_string", "svpf_diff_tape_floats", "svpf_diff_forward", "svpf_diff_backward",
-    "svpf_tile_occupancy_query",
+    "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
 )
 
 _lib = None

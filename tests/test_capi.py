@@ -72,12 +72,13 @@ def test_missing_library_fails_loudly(tmp_path):
 
 def test_record_layouts_match_c(tmp_path):
     """numpy record dtypes of the batched API against csrc/svpf_types.h."""
-    from svpf_amd import batched, diff
+    from svpf_amd import batched, diff, twofactor
 
     csrc = os.path.join(ROOT, "newton-stein-variational-particle-filter_amd", "csrc")
     checks = [("svpf_fparams", batched.FPARAMS_DTYPE), ("svpf_fstate", batched.FSTATE_DTYPE),
               ("svpf_fdesc", batched.FDESC_DTYPE), ("svpf_dconfig", diff.DCONFIG_DTYPE),
-              ("svpf_ddesc", diff.DDESC_DTYPE)]
+              ("svpf_ddesc", diff.DDESC_DTYPE), ("svpf_d2config", twofactor.D2CONFIG_DTYPE),
+              ("svpf_d2desc", twofactor.D2DESC_DTYPE)]
     lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "svpf_types.h"', "int main(void) {"]
     for cname, dt in checks:
         lines.append(f'printf("{cname}.size %zu\\n", sizeof({cname}));')

@@ -1,0 +1,111 @@
+"""Two-factor (2-D) form on the GPU through the C ABI: the forward against the
+float64 dense restatement (oracle/torch_svpf2d.py), the hand-written adjoint
+against its autograd gradient to 1e-3 relative, the gradient's independence
+of the checkpoint interval, the counter generator, and a short Adam
+calibration that lowers the loss."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import torch_svpf2d
+from svpf_amd import data
+from svpf_amd.twofactor import O_FMEAN, O_HMEAN, O_PRED_LL, O_SMEAN, TwoFactorBatch, calibrate, predictive_loglik_2d
+
+pytestmark = pytest.mark.gpu
+
+THETAS = [(-1.0, 0.80, 0.30, 0.995, 0.08, 5.0), (-0.6, 0.70, 0.40, 0.98, 0.12, 8.0),
+          (-1.4, 0.85, 0.25, 0.99, 0.10, 12.0)]
+# Gradient checks use well-conditioned filters: with a slow factor as tight as sigma_s = 0.08,
+# rho_s = 0.995 RMSProp's v sits near 0 on some series, and eps phi / sqrt(v + 1e-6) amplifies
+# FP32 rounding until the FP32 gradient differs from the FP64 one by percent (a float32
+# emulation of this adjoint against float64 shows the same).
+GRAD_THETAS = [(-1.0, 0.80, 0.30, 0.97, 0.15, 5.0), (-0.6, 0.70, 0.40, 0.95, 0.20, 8.0),
+               (-1.4, 0.85, 0.25, 0.96, 0.18, 12.0)]
+
+
+def inputs(n, T, S, thetas, seed=0):
+    ys, x0s, zs, xis = [], [], [], []
+    for b, (mu, rf, sf, rs, ss, nu) in enumerate(thetas):
+        y, _, _, _ = data.two_factor_series(T, mu=mu, rho_f=rf, sigma_f=sf, rho_s=rs, sigma_s=ss, nu=nu,
+                                            seed=seed + 10 * b)
+        rng = np.random.default_rng(seed + 10 * b + 1)
+        ys.append(y)
+        x0s.append(np.stack([rng.standard_normal(n) * sf / np.sqrt(1 - rf * rf),
+                             rng.standard_normal(n) * ss / np.sqrt(1 - rs * rs)]).astype(np.float32))
+        zs.append(rng.standard_normal((T, 2, n)).astype(np.float32))
+        xis.append(rng.standard_normal((T, S, 2, n)).astype(np.float32))
+    return np.stack(ys), np.stack(x0s), np.stack(zs), np.stack(xis)
+
+
+def make(n, T, S, thetas, C, seed=0, sign=False):
+    y, x0, z, xi = inputs(n, T, S, thetas, seed)
+    bt = TwoFactorBatch(y, x0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi}, paper_sign=sign)
+    return bt, (y, x0, z, xi)
+
+
+def torch_ref(th, y, x0, z, xi, S, sign=0, grad=False):
+    dt = torch.float64
+    theta = [torch.tensor(v, dtype=dt, requires_grad=grad) for v in th]
+    cfg = dict(torch_svpf2d.DEFAULT_CFG, S=S, paper_sign=sign)
+    pl, x, v = torch_svpf2d.run(theta, torch.tensor(y, dtype=dt), torch.tensor(z, dtype=dt).transpose(1, 2),
+                                torch.tensor(xi, dtype=dt).transpose(2, 3), torch.tensor(x0, dtype=dt).T, cfg)
+    return theta, pl, x
+
+
+@pytest.mark.parametrize("n", [48, 300, 1024, 2048])
+def test_forward_matches_torch_restatement(torch_cuda, n):
+    T, S = 6, 5
+    bt, (y, x0, z, xi) = make(n, T, S, THETAS, C=4)
+    out = bt.forward(torch.tensor(THETAS, device="cuda")).cpu().numpy()
+    xT = bt.xT.cpu().numpy()
+    for b, th in enumerate(THETAS):
+        _, pl, x = torch_ref(th, y[b], x0[b], z[b], xi[b], S)
+        assert np.max(np.abs(out[b, :, O_PRED_LL] - pl.numpy())) <= 1e-4 * max(1.0, np.abs(pl.numpy()).max()), b
+        xm = x.numpy()
+        assert np.max(np.abs(out[b, -1, O_FMEAN] - xm[:, 0].mean())) <= 1e-4, b
+        assert np.max(np.abs(out[b, -1, O_SMEAN] - xm[:, 1].mean())) <= 1e-4, b
+        assert abs(out[b, -1, O_HMEAN] - (th[0] + xm.sum(1).mean())) <= 1e-4, b
+        assert np.max(np.abs(xT[b].T - xm)) <= 1e-3, b
+
+
+def rel_err(g, ref):
+    return np.max(np.abs(g - ref)) / max(1e-12, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("sign", [0, 1])
+def test_adjoint_matches_torch_autograd(torch_cuda, sign):
+    n, T, S, C = 40, 8, 5, 3
+    bt, (y, x0, z, xi) = make(n, T, S, GRAD_THETAS, C, seed=3, sign=bool(sign))
+    theta = torch.tensor(GRAD_THETAS, device="cuda", requires_grad=True)
+    (-predictive_loglik_2d(theta, bt).sum()).backward()
+    g = theta.grad.cpu().numpy()
+    for b, th in enumerate(GRAD_THETAS):
+        tt, pl, _ = torch_ref(th, y[b], x0[b], z[b], xi[b], S, sign, grad=True)
+        (-pl.sum()).backward()
+        ref = np.array([t.grad.item() for t in tt])
+        assert rel_err(g[b], ref) <= 1e-3, (b, g[b], ref)
+
+
+def test_gradient_independent_of_checkpoint_interval(torch_cuda):
+    n, T, S = 128, 7, 5
+    grads = []
+    for C in (1, 3, 7):
+        bt, _ = make(n, T, S, GRAD_THETAS, C, seed=7)
+        theta = torch.tensor(GRAD_THETAS, device="cuda", requires_grad=True)
+        (-predictive_loglik_2d(theta, bt).sum()).backward()
+        grads.append(theta.grad.cpu().numpy())
+    for g in grads[1:]:
+        assert np.array_equal(g, grads[0])
+
+
+def test_counter_generator_and_calibration_lowers_loss(torch_cuda):
+    n, T, S = 128, 60, 5
+    true = THETAS[0]
+    y, x0, _, _ = inputs(n, T, S, [true] * 4, seed=11)
+    bt = TwoFactorBatch(y, x0, n_stein=S, checkpoint_every=16)
+    start = torch.tensor([true] * 4, device="cuda")
+    start[:, 2] *= 1.5  # perturbed sigma_f
+    start[:, 0] += 0.4  # perturbed mu
+    th, hist = calibrate(bt, start, epochs=25, lr=2e-2)
+    assert np.isfinite(hist).all()
+    assert (hist[-1] < hist[0]).all()
