@@ -272,6 +272,86 @@ def cpu_diff_baseline(theta, ys, h0, ticks=2):
     return fwd, fb
 
 
+TWO_FACTOR_THETA = (-1.0, 0.80, 0.30, 0.995, 0.08, 5.0)  # mu, rho_f, sigma_f, rho_s, sigma_s, nu
+
+
+def twofactor_workload(B, N, T, seed=5):
+    """B two-factor filters of N particles; each filter's own series from the generating theta."""
+    from svpf_amd import data
+
+    mu, rf, sf, rs, ss, nu = TWO_FACTOR_THETA
+    ys = np.zeros((B, T), np.float32)
+    x0 = np.zeros((B, 2, N), np.float32)
+    rng = np.random.default_rng(seed)
+    for b in range(B):
+        ys[b] = data.two_factor_series(T, mu=mu, rho_f=rf, sigma_f=sf, rho_s=rs, sigma_s=ss, nu=nu, seed=seed + b)[0]
+        x0[b, 0] = rng.standard_normal(N) * sf / np.sqrt(1 - rf * rf)
+        x0[b, 1] = rng.standard_normal(N) * ss / np.sqrt(1 - rs * rs)
+    return ys, x0
+
+
+def time_twofactor(torch, B=1024, N=2048, T=32, C=8):
+    """Two-factor filter at the cfg shape: forward and forward + adjoint (checkpoints every C ticks)."""
+    from svpf_amd.twofactor import TwoFactorBatch
+
+    ys, x0 = twofactor_workload(B, N, T)
+    bt = TwoFactorBatch(ys, x0, n_stein=S_ITERS, checkpoint_every=C)
+    th = torch.tensor([TWO_FACTOR_THETA] * B, device="cuda")
+    g = -torch.ones((B, T), device="cuda")
+    bt.forward(th)
+    bt.backward(g)
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(s)
+    bt.forward(th)
+    ev[1].record(s)
+    bt.backward(g)
+    ev[2].record(s)
+    torch.cuda.synchronize()
+    fwd, bwd = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    assert torch.isfinite(bt.gtheta).all() and torch.isfinite(bt.out).all()
+    del bt
+    torch.cuda.empty_cache()
+    flops = float(N) ** 2 * 39 * S_ITERS * B * T  # 2-D pair work: 21 flops transport + 18 prior per pair-iteration
+    return {"workload": f"two-factor B={B} N={N} T={T} S={S_ITERS}, checkpoints every {C} ticks",
+            "filter_steps_per_s": B * T / (fwd * 1e-3), "fwd_ms": fwd, "fwd_bwd_ms": fwd + bwd,
+            "fwd_bwd_over_fwd": (fwd + bwd) / fwd, "fwd_tflops": flops / (fwd * 1e-3) / 1e12}
+
+
+def calibration_demo(torch, B=128, N=256, T=600, epochs=200, lr=1e-2):
+    """Adam on one theta shared by B two-factor filters (each filter runs its own series
+    drawn from the generating theta) from a perturbed start; reports the recovered theta."""
+    from svpf_amd.twofactor import THETA_NAMES, TwoFactorBatch, calibrate
+
+    ys, x0 = twofactor_workload(B, N, T, seed=77)
+    bt = TwoFactorBatch(ys, x0, n_stein=S_ITERS, checkpoint_every=32)
+    true = np.array(TWO_FACTOR_THETA, np.float32)
+    start = true.copy()
+    start[0] += 0.5
+    start[1] = 0.6
+    start[2] *= 1.5
+    start[3] = 0.98
+    start[4] *= 1.5
+    start[5] = 10.0
+    t0 = time.perf_counter()
+    th, hist = calibrate(bt, torch.tensor(start, device="cuda"), epochs=epochs, lr=lr, shared=True)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    est = th.cpu().numpy()
+    # the summed loss at the generating, starting and fitted theta (same series and noise)
+    loss_at = {}
+    for name, v in (("true", true), ("start", start), ("estimate", est)):
+        pl = bt.forward(torch.tensor(np.tile(v, (B, 1)), device="cuda"))[:, :, 0]
+        loss_at[name] = float(-pl.sum())
+    return {"workload": f"one theta shared by B={B} filters (own series each), N={N}, T={T}, S={S_ITERS}, "
+                        f"Adam lr {lr}, {epochs} epochs",
+            "true": dict(zip(THETA_NAMES, true.tolist())), "start": dict(zip(THETA_NAMES, start.tolist())),
+            "estimate": dict(zip(THETA_NAMES, est.tolist())),
+            "loss_first": float(hist[0].sum()), "loss_last": float(hist[-1].sum()), "loss_at": loss_at,
+            "wall_s": wall}
+
+
 def profile_only(torch, what, B, K, W):
     """One workload for a profiler run; prints what ran."""
     if what == "dominant":
@@ -282,6 +362,10 @@ def profile_only(torch, what, B, K, W):
         cfgs, ys, h0s, n = build_workload(B, W + K + 1)
         ms, flops, nf, groups = time_dominant_kernel(torch, cfgs, ys, h0s, n, W, K)
         log(f"copy 256 MiB; dominant group: {nf} filters, {K} ticks, {ms:.2f} ms, {flops / ms / 1e9:.2f} TFLOP/s")
+    elif what == "calib":
+        print(json.dumps(calibration_demo(torch)), flush=True)
+    elif what == "twofactor":
+        log(json.dumps(time_twofactor(torch)))
     elif what == "diff":
         theta, dys, dh0 = diff_workload()
         fwd, bwd, _ = time_diff(torch, theta, dys, dh0, 32)
@@ -298,10 +382,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--B", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
-    ap.add_argument("--only", choices=["dominant", "diff", "single"], default=None,
+    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib"], default=None,
                     help="run one workload once and exit (for profiler runs): the N=2048 tile group of the "
                          "batched workload preceded by a 256 MiB device copy (traffic calibration), the "
-                         "differentiable filter, or the single N=512 filter")
+                         "differentiable filter, the single N=512 filter, the two-factor filter, or the "
+                         "two-factor calibration (prints its JSON)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: filters are independent and this build runs on one MI355X; "
@@ -369,6 +454,8 @@ def main():
                              "sample": "dense torch float32 restatement, 1 filter N=1024, 2 ticks, autograd"},
         }
         extras["fwd_bwd_us_per_step"] = (fwd32 + bwd32) * 1e3 / T_d
+        log("two-factor filter B=1024 N=2048")
+        extras["twofactor"] = time_twofactor(torch)
     log("cpu baseline (dense torch restatement)")
     cpu_rate, cpu_bucket = cpu_baseline(cfgs, ys, h0s, n, ticks=8)
     log("secondary reference (FP64 oracle, one thread)")

@@ -171,10 +171,12 @@ def to_constrained(u):
                         torch.exp(u[..., 4]), 2.0 + torch.exp(u[..., 5])], -1)
 
 
-def calibrate(batch, theta0, epochs=200, lr=1e-2, fixed=None, log=None):
-    """Adam on the unconstrained parameters of every filter, loss = -sum_t pred_ll
-    per filter (filters are independent).  fixed: names of parameters held at theta0.
-    Returns (theta [B, 6], loss history [epochs, B])."""
+def calibrate(batch, theta0, epochs=200, lr=1e-2, fixed=None, shared=False, log=None):
+    """Adam on the unconstrained parameters, loss = -sum_t pred_ll per filter.
+    shared=False: every filter fits its own theta (theta0 [B, 6]); shared=True: one
+    theta (theta0 [6]) for all filters, whose losses add up (series drawn from one
+    generating theta).  fixed: names of parameters held at theta0.
+    Returns (theta [B, 6] or [6], loss history [epochs, B])."""
     u = to_unconstrained(torch.as_tensor(theta0, device=batch.y.device)).clone().requires_grad_(True)
     mask = torch.ones(6, device=batch.y.device)
     for name in fixed or ():
@@ -183,7 +185,10 @@ def calibrate(batch, theta0, epochs=200, lr=1e-2, fixed=None, log=None):
     hist = []
     for ep in range(epochs):
         opt.zero_grad()
-        pl = predictive_loglik_2d(to_constrained(u), batch)
+        th = to_constrained(u)
+        if shared:
+            th = th.expand(batch.B, 6)
+        pl = predictive_loglik_2d(th, batch)
         loss = -pl.sum(1)
         loss.sum().backward()
         u.grad *= mask
