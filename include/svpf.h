@@ -38,6 +38,11 @@ extern "C" {
 #define SVPF_H_MAX 5.0f
 #define SVPF_SMOOTH_MAX_LAG 8
 
+/* Largest n_particles of one filter.  Up to the single-workgroup capacity
+ * (svpf_tile_capacity_query) a filter runs in one workgroup's LDS; larger
+ * filters run on the grid path, a per-tick sequence of grid-wide kernels. */
+#define SVPF_GRID_MAX_PARTICLES 16384
+
 /* include/svpf.cuh:84-90 */
 #define SVPF_STEIN_SIGN_LEGACY 0
 #define SVPF_STEIN_SIGN_PAPER 1
@@ -194,7 +199,8 @@ typedef struct {
     float mu_estimate;
 } SVPFResult;
 
-/* include/svpf.cuh:573 -- NULL on any failure (reference: malloc failure only). */
+/* include/svpf.cuh:573 -- NULL on any failure (reference: malloc failure only),
+ * including n_particles above SVPF_GRID_MAX_PARTICLES. */
 SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t stream);
 /* include/svpf.cuh:578 */
 void svpf_destroy(SVPFState* state);
@@ -261,6 +267,18 @@ int svpf_get_last_outputs(const SVPFState* state, float out8[8]);
  * on one XCD, 1 spreads them.  Pass -1 for the defaults. */
 int svpf_set_coop_shape(SVPFState* state, int lanes_per_row, int xcd_stride);
 int svpf_get_coop_shape(const SVPFState* state, int* lanes_per_row, int* workgroups, int* xcd_stride);
+
+/* No reference counterpart: the kernel path serving one filter.  The grid path
+ * (a per-tick sequence of grid-wide kernels) is forced on with enable = 1 at any
+ * n_particles; enable = 0 returns to the single-workgroup / cooperative kernels,
+ * and fails with SVPF_ERR_UNSUPPORTED when n_particles exceeds the
+ * single-workgroup capacity (the grid path is then the only one).
+ * svpf_get_path returns SVPF_PATH_* of the next step, or -1 for a bad state. */
+#define SVPF_PATH_SINGLE 0
+#define SVPF_PATH_COOP 1
+#define SVPF_PATH_GRID 2
+int svpf_set_grid_path(SVPFState* state, int enable);
+int svpf_get_path(const SVPFState* state);
 
 /* include/svpf.cuh:626-632 */
 static inline void svpf_set_stein_sign_mode(SVPFState* state, int mode) {

@@ -76,6 +76,9 @@ struct svpf_impl {
     float med_r2 = 0.0f;  // last median pairwise r^2 (a search seed, not a result)
     bool captured_coop = false;
     bool coop_graph_failed = false;
+    // grid path: filters too large for one workgroup's LDS
+    bool grid = false;
+    float* dgrid_ws = nullptr;
 
     svpf_fparams* hp() { return reinterpret_cast<svpf_fparams*>(hblock + kBlockParams); }
     svpf_fstate* hs() { return reinterpret_cast<svpf_fstate*>(hblock + kBlockFstate); }
@@ -324,11 +327,14 @@ int check_supported(SVPFState* s) {
 // for it (use_persistent_kernel) and the features it implements suffice.
 bool use_coop(const SVPFState* s) {
     const svpf_impl* I = s->impl;
-    return s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop;
+    return !I->grid && s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop;
 }
 
 hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
     svpf_impl* I = s->impl;
+    if (I->grid)
+        return svpf_launch_grid(I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->d_lw(), I->dgrid_ws,
+                                I->n, s->stein_max_steps, t0, t1, cs);
     if (use_coop(s))
         return svpf_launch_coop(I->coop_cs, I->coop_G, I->coop_stride, I->dp(), I->ds(), I->dd(), I->d_h(),
                                 I->d_hp(), I->d_v(), I->xbuf, I->n, t0, t1, 1, cs);
@@ -517,7 +523,9 @@ extern "C" {
 SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t stream) {
     if (n_particles < 1 || nu <= 0.0f) return nullptr;
     int tile = -1, bs = 256;
-    if (!choose_tile(n_particles, &tile, &bs)) return nullptr;
+    // one workgroup per filter while its LDS holds the filter, else the grid path
+    const bool grid = !choose_tile(n_particles, &tile, &bs);
+    if (grid && (n_particles > SVPF_GRID_MAX_PARTICLES || svpf_grid_init() != hipSuccess)) return nullptr;
     if (svpf_kernels_init() != hipSuccess) return nullptr;
     SVPFState* s = static_cast<SVPFState*>(std::calloc(1, sizeof(SVPFState)));
     if (!s) return nullptr;
@@ -530,7 +538,9 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
     I->n = n_particles;
     I->tile = tile;
     I->bs = bs;
+    I->grid = grid;
     bool ok = hipMalloc(&I->dblock, kBlockBytes) == hipSuccess;
+    if (grid) ok = ok && hipMalloc(&I->dgrid_ws, svpf_grid_workspace_floats(n_particles) * sizeof(float)) == hipSuccess;
     ok = ok && hipHostMalloc(&I->hblock, kBlockBytes, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipMalloc(&I->dpool, 5 * (size_t)n_particles * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&I->dorder, sizeof(int)) == hipSuccess;
@@ -546,7 +556,7 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
         ok = ok && hipMemset(I->dblock, 0, kBlockBytes) == hipSuccess;
         ok = ok && hipMemset(I->xbuf, 0, xwords * sizeof(uint64_t)) == hipSuccess;
     }
-    if (ok) choose_coop_shape(I, -1, -1);
+    if (ok && !grid) choose_coop_shape(I, -1, -1);
     if (!ok) {
         svpf_destroy(s);
         return nullptr;
@@ -653,6 +663,7 @@ void svpf_destroy(SVPFState* s) {
         if (I->dpool) hipFree(I->dpool);
         if (I->dorder) hipFree(I->dorder);
         if (I->xbuf) hipFree(I->xbuf);
+        if (I->dgrid_ws) hipFree(I->dgrid_ws);
         if (I->dseq_out) hipFree(I->dseq_out);
         if (I->dseq_y) hipFree(I->dseq_y);
         delete I;
@@ -989,6 +1000,7 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
 int svpf_set_coop_shape(SVPFState* s, int lanes_per_row, int xcd_stride) {
     if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
     svpf_impl* I = s->impl;
+    if (I->grid) return set_err(s, SVPF_ERR_UNSUPPORTED);
     svpf_impl trial = {};
     trial.n = I->n;
     if (!choose_coop_shape(&trial, lanes_per_row, xcd_stride)) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
@@ -998,6 +1010,35 @@ int svpf_set_coop_shape(SVPFState* s, int lanes_per_row, int xcd_stride) {
     I->coop_ok = true;
     svpf_graph_invalidate(s);
     return set_err(s, SVPF_OK);
+}
+
+int svpf_set_grid_path(SVPFState* s, int enable) {
+    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    svpf_impl* I = s->impl;
+    if (!enable) {
+        if (I->tile < 0) return set_err(s, SVPF_ERR_UNSUPPORTED);
+        if (I->grid) svpf_graph_invalidate(s);
+        I->grid = false;
+        return set_err(s, SVPF_OK);
+    }
+    if (I->n < 2 || I->n > SVPF_GRID_MAX_PARTICLES) return set_err(s, SVPF_ERR_UNSUPPORTED);
+    hipError_t e = svpf_grid_init();
+    if (e == hipSuccess && !I->dgrid_ws)
+        e = hipMalloc(&I->dgrid_ws, svpf_grid_workspace_floats(I->n) * sizeof(float));
+    if (e != hipSuccess) {
+        I->dgrid_ws = nullptr;
+        I->last_hip = e;
+        return set_err(s, e == hipErrorOutOfMemory ? SVPF_ERR_OUT_OF_MEMORY : SVPF_ERR_HIP);
+    }
+    if (!I->grid) svpf_graph_invalidate(s);
+    I->grid = true;
+    return set_err(s, SVPF_OK);
+}
+
+int svpf_get_path(const SVPFState* s) {
+    if (!s || !s->impl) return -1;
+    if (s->impl->grid) return SVPF_PATH_GRID;
+    return use_coop(s) ? SVPF_PATH_COOP : SVPF_PATH_SINGLE;
 }
 
 int svpf_get_coop_shape(const SVPFState* s, int* lanes_per_row, int* workgroups, int* xcd_stride) {

@@ -468,10 +468,10 @@ SVPF_DEV void row_search(const float* s, int n, float si, int a, int b, const ui
 // bracketed the bracket is split evenly.  Once at
 // most `cmax` candidate pairs remain they are gathered into `cand` and the
 // rank is read off by counting.  Scratch (LDS): plo, phi (n ints each), cand
-// (cmax floats), red (48 + BS/64 + 1 words).  The first KR row slots of each thread keep
+// (cmax floats), red (48 + BS/64 + 1 words); Idx may be a 16-bit type when n < 65536.  The first KR row slots of each thread keep
 // their search results in registers; later slots search again.
-template <int BS, int KR>
-SVPF_DEV float median_pairwise_r2(float* s, int n, float est, int* plo, int* phi, float* cand, int cmax,
+template <int BS, int KR, typename Idx = int>
+SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi, float* cand, int cmax,
                                   uint32_t* red) {
     constexpr int W = 7;
     constexpr int NW = BS / kWave;

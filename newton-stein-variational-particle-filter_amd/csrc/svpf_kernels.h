@@ -71,3 +71,18 @@ hipError_t svpf_launch_diff2d(int backward, const svpf_d2config* C, const svpf_d
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+// Grid path for one large filter (n up to 16384): per tick a fixed sequence of
+// grid-wide kernels on `stream`, with the Stein budget decided on the device.
+size_t svpf_grid_workspace_floats(int n);
+size_t svpf_grid_lds_bytes(int n);
+hipError_t svpf_grid_init(void);
+hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, float* h, float* hp,
+                            float* v, float* lw, float* ws, int n, int stein_max, int t0, int t1,
+                            hipStream_t stream);
+#ifdef __cplusplus
+}
+#endif

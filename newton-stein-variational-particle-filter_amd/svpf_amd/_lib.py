@@ -86,6 +86,7 @@ EXPORTED = (
     "svpf_lds_bytes_query", "svpf_version", "svpf_set_coop_shape", "svpf_get_coop_shape",
     "svpf_last_error_string", "svpf_diff_tape_floats", "svpf_diff_forward", "svpf_diff_backward",
     "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
+    "svpf_set_grid_path", "svpf_get_path",
 )
 
 _lib = None
@@ -159,6 +160,10 @@ def load(path: str | None = None):
     L.svpf_set_coop_shape.restype = c_int
     L.svpf_set_coop_shape.argtypes = [SP, c_int, c_int]
     L.svpf_get_coop_shape.restype = c_int
+    L.svpf_set_grid_path.restype = c_int
+    L.svpf_set_grid_path.argtypes = [SP, c_int]
+    L.svpf_get_path.restype = c_int
+    L.svpf_get_path.argtypes = [SP]
     L.svpf_get_coop_shape.argtypes = [SP, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
     L.svpf_version.restype = ctypes.c_char_p
     L.svpf_version.argtypes = []

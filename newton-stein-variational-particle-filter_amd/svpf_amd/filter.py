@@ -24,7 +24,9 @@ class Filter:
 
     def __init__(self, cfg: dict, stream=None, path: str = "auto", coop_shape=None):
         """path: "auto" (the library default: cooperative multi-workgroup kernel
-        for n >= 256), "coop" or "single" (one workgroup per filter)."""
+        for n >= 256, the grid path above the single-workgroup capacity),
+        "coop", "single" (one workgroup per filter) or "grid" (per-tick
+        sequence of grid-wide kernels, any n >= 2)."""
         self.L = _lib.load()
         self.cfg = dict(cfg)
         n = int(cfg["n"])
@@ -40,10 +42,18 @@ class Filter:
             self.state.use_persistent_kernel = 0
         elif path == "coop":
             self.state.use_persistent_kernel = 1
+        elif path == "grid":
+            rc = self.L.svpf_set_grid_path(self.sp, 1)
+            if rc:
+                raise RuntimeError(f"svpf_set_grid_path(n={n}) rejected ({rc})")
         if coop_shape is not None:
             rc = self.L.svpf_set_coop_shape(self.sp, int(coop_shape[0]), int(coop_shape[1]))
             if rc:
                 raise RuntimeError(f"svpf_set_coop_shape{tuple(coop_shape)} rejected ({rc})")
+
+    def path(self):
+        """"single", "coop" or "grid": the kernel path of the next step."""
+        return {0: "single", 1: "coop", 2: "grid"}[self.L.svpf_get_path(self.sp)]
 
     def coop_shape(self):
         cs, g, st = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()

@@ -104,16 +104,34 @@ double svpfo_log_w(double h, double y, double nu, double st_const) {
     return st_const - 0.5 * h - (nu + 1.0) * 0.5 * log1p(a);
 }
 
-static int cmp_double(const void* a, const void* b) {
-    double x = *(const double*)a, y = *(const double*)b;
-    return (x > y) - (x < y);
-}
-static int cmp_float(const void* a, const void* b) {
-    float x = *(const float*)a, y = *(const float*)b;
-    return (x > y) - (x < y);
-}
+/* k-th smallest (0-based) of a[0, m), a reordered: iterative quickselect with
+ * a median-of-three pivot and a three-way partition (keys equal to the pivot
+ * end the search), O(m) expected.  Same value as a[k] of the sorted array. */
+#define SVPFO_SELECT(NAME, T)                                                       \
+    static T NAME(T* a, size_t m, size_t k) {                                       \
+        size_t lo = 0, hi = m;                                                      \
+        while (hi - lo > 1) {                                                       \
+            T x = a[lo], y = a[lo + (hi - lo) / 2], z = a[hi - 1];                  \
+            T mn = x < y ? x : y, mx = x < y ? y : x;                               \
+            T p = mx < z ? mx : z;                                                  \
+            p = mn > p ? mn : p;                                                    \
+            size_t lt = lo, i = lo, gt = hi;                                        \
+            while (i < gt) {                                                        \
+                T v = a[i];                                                         \
+                if (v < p) { a[i] = a[lt]; a[lt] = v; ++lt; ++i; }                  \
+                else if (v > p) { --gt; a[i] = a[gt]; a[gt] = v; }                  \
+                else ++i;                                                           \
+            }                                                                       \
+            if (k < lt) hi = lt;                                                    \
+            else if (k >= gt) lo = gt;                                              \
+            else return p;                                                          \
+        }                                                                           \
+        return a[lo];                                                               \
+    }
+SVPFO_SELECT(select_kth_d, double)
+SVPFO_SELECT(select_kth_f, float)
 
-/* Lower median of the pairwise squared distances, by full sort. */
+/* Lower median (rank (M-1)/2 of the M = n(n-1)/2 values) of the pairwise squared distances. */
 double svpfo_median_r2(const double* h, int n) {
     if (n < 2) return 0.0;
     size_t m = (size_t)n * (size_t)(n - 1) / 2, k = 0;
@@ -121,8 +139,7 @@ double svpfo_median_r2(const double* h, int n) {
     if (!r) return NAN;
     for (int i = 0; i < n; ++i)
         for (int j = i + 1; j < n; ++j) { double d = h[i] - h[j]; r[k++] = d * d; }
-    qsort(r, m, sizeof(double), cmp_double);
-    double med = r[(m - 1) / 2];
+    double med = select_kth_d(r, m, (m - 1) / 2);
     free(r);
     return med;
 }
@@ -136,8 +153,7 @@ float svpfo_median_r2_f32(const float* h, int n) {
     if (!r) return NAN;
     for (int i = 0; i < n; ++i)
         for (int j = i + 1; j < n; ++j) { volatile float d = h[i] - h[j]; volatile float q = d * d; r[k++] = q; }
-    qsort(r, m, sizeof(float), cmp_float);
-    float med = r[(m - 1) / 2];
+    float med = select_kth_f(r, m, (m - 1) / 2);
     free(r);
     return med;
 }
