@@ -14,7 +14,9 @@ N=512 device-loop microseconds per step and the interactive per-tick latency;
 the roofline of the dominant kernel (the N=2048 tile group) against the
 FP32 vector peak; the differentiable filter (B=256, N=1024, T=500): forward
 and forward + adjoint with checkpoints every 32 ticks and with the full tape;
-the dense PyTorch restatement timed on the host cores as CPU baseline.
+the dense PyTorch restatement timed on the host cores as CPU baseline; the
+full-feature single filter at N=4096 and N=16384 (regime-shift series, T=2000)
+with the FP64 oracle as its CPU baseline.
 """
 import argparse
 import json
@@ -148,6 +150,60 @@ def time_single(torch, T=1000, warm=50):
     f.close()
     lat = np.array(lat[10:]) * 1e6
     return us_device, float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+
+
+def time_large(torch, n, T=2000):
+    """One full-feature filter (MIM, asymmetric rho, EKF guide, SVLD, KSD-adaptive
+    beta, early stop, at most 8 Stein iterations) on the regime-shift series:
+    device loop over T ticks (timed), then the same series tick by tick for the
+    executed Stein iterations."""
+    from svpf_amd import config, data
+    from svpf_amd.filter import Filter
+
+    cfg = config.full_feature_config(n=n)
+    y, _ = data.regime_shift_series(T=T, seed=0)
+    h0 = data.initial_particles(n, -1.0, 0.98, 0.2, seed=1)
+    f = Filter(cfg)
+    path = f.path()
+    dy = torch.tensor(y, device="cuda")
+    ll = torch.zeros(T, device="cuda")
+    f.set_particles(h0)
+    f.run_device(dy[:20], ll[:20])  # warm-up
+    f.set_particles(h0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    f.run_device(dy, ll)
+    torch.cuda.synchronize()
+    us = (time.perf_counter() - t0) / T * 1e6
+    assert torch.isfinite(ll).all()
+    f.set_particles(h0)
+    steps = np.zeros(T)
+    yp = 0.0
+    for t in range(T):
+        f.step(float(y[t]), yp)
+        steps[t] = f.last_outputs()[6]
+        yp = float(y[t])
+    f.close()
+    it = float(steps.mean())
+    flops = float(n) ** 2 * (23 * it + 9)
+    return {"us_per_step": us, "path": path, "executed_iters_mean": it, "tflops": flops / (us * 1e-6) / 1e12}
+
+
+def cpu_large_baseline(n=4096, ticks=2):
+    """The FP64 oracle (one thread) on the full-feature filter: seconds per tick."""
+    import oracle
+    from svpf_amd import config, data
+
+    cfg = config.full_feature_config(n=n)
+    y, _ = data.regime_shift_series(T=ticks, seed=0)
+    h0 = data.initial_particles(n, -1.0, 0.98, 0.2, seed=1)
+    f = oracle.OracleFilter(cfg, h0)
+    t0 = time.perf_counter()
+    yp = 0.0
+    for t in range(ticks):
+        f.tick(float(y[t]), yp, seed=42)
+        yp = float(y[t])
+    return (time.perf_counter() - t0) / ticks
 
 
 def cpu_threads():
@@ -362,6 +418,8 @@ def profile_only(torch, what, B, K, W):
         cfgs, ys, h0s, n = build_workload(B, W + K + 1)
         ms, flops, nf, groups = time_dominant_kernel(torch, cfgs, ys, h0s, n, W, K)
         log(f"copy 256 MiB; dominant group: {nf} filters, {K} ticks, {ms:.2f} ms, {flops / ms / 1e9:.2f} TFLOP/s")
+    elif what == "large":
+        log(json.dumps(time_large(torch, 16384, T=K)))
     elif what == "calib":
         print(json.dumps(calibration_demo(torch)), flush=True)
     elif what == "twofactor":
@@ -382,11 +440,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--B", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
-    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib"], default=None,
+    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "large"], default=None,
                     help="run one workload once and exit (for profiler runs): the N=2048 tile group of the "
                          "batched workload preceded by a 256 MiB device copy (traffic calibration), the "
-                         "differentiable filter, the single N=512 filter, the two-factor filter, or the "
-                         "two-factor calibration (prints its JSON)")
+                         "differentiable filter, the single N=512 filter, the two-factor filter, the "
+                         "two-factor calibration (prints its JSON), or the full-feature N=16384 filter "
+                         "over --steps ticks")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: filters are independent and this build runs on one MI355X; "
@@ -456,6 +515,15 @@ def main():
         extras["fwd_bwd_us_per_step"] = (fwd32 + bwd32) * 1e3 / T_d
         log("two-factor filter B=1024 N=2048")
         extras["twofactor"] = time_twofactor(torch)
+        log("full-feature single filters N=4096 and N=16384, T=2000")
+        large = {str(N): time_large(torch, N) for N in (4096, 16384)}
+        log("cpu baseline of the full-feature filter (FP64 oracle, N=4096, one thread)")
+        large["cpu_baseline"] = {"s_per_step_n4096": cpu_large_baseline(), "cores": 1, "kind": "port",
+                                 "sample": "FP64 C oracle, full-feature filter N=4096, 2 ticks"}
+        large["workload"] = ("one filter, regime-shift series T=2000 (mu -1 -> 1.5 at t=1000, jumps p=0.02 of +2, "
+                             "rho up 0.90 / down 0.98, sigma_z 0.2, nu 4), MIM, EKF guide, SVLD T=1, "
+                             "KSD-adaptive beta, early stop, at most 8 Stein iterations")
+        extras["large_filter"] = large
     log("cpu baseline (dense torch restatement)")
     cpu_rate, cpu_bucket = cpu_baseline(cfgs, ys, h0s, n, ticks=8)
     log("secondary reference (FP64 oracle, one thread)")

@@ -268,16 +268,26 @@ int svpf_get_last_outputs(const SVPFState* state, float out8[8]);
 int svpf_set_coop_shape(SVPFState* state, int lanes_per_row, int xcd_stride);
 int svpf_get_coop_shape(const SVPFState* state, int* lanes_per_row, int* workgroups, int* xcd_stride);
 
-/* No reference counterpart: the kernel path serving one filter.  The grid path
- * (a per-tick sequence of grid-wide kernels) is forced on with enable = 1 at any
- * n_particles; enable = 0 returns to the single-workgroup / cooperative kernels,
- * and fails with SVPF_ERR_UNSUPPORTED when n_particles exceeds the
- * single-workgroup capacity (the grid path is then the only one).
- * svpf_get_path returns SVPF_PATH_* of the next step, or -1 for a bad state. */
+/* No reference counterpart: the kernel path serving one filter.
+ *   SVPF_PATH_SINGLE  one workgroup holds the filter in LDS;
+ *   SVPF_PATH_COOP    the cooperative multi-workgroup kernel (use_persistent_kernel);
+ *   SVPF_PATH_GRID    a per-tick sequence of grid-wide kernels (2 <= n_particles
+ *                     <= SVPF_GRID_MAX_PARTICLES; the only path above the
+ *                     single-workgroup capacity).
+ * svpf_set_grid_path: SVPF_GRID_AUTO (default) picks by n_particles and the
+ * features in use -- cooperative kernel for 256 <= n < 1024 where it applies,
+ * grid path from n = 1024 up and for 256 <= n < 1024 otherwise, one workgroup
+ * below 256; SVPF_GRID_ON forces the grid path; SVPF_GRID_OFF forbids it
+ * (SVPF_ERR_UNSUPPORTED when n_particles exceeds the single-workgroup
+ * capacity).  svpf_get_path returns the SVPF_PATH_* of the next step, -1 for a
+ * bad state. */
 #define SVPF_PATH_SINGLE 0
 #define SVPF_PATH_COOP 1
 #define SVPF_PATH_GRID 2
-int svpf_set_grid_path(SVPFState* state, int enable);
+#define SVPF_GRID_AUTO (-1)
+#define SVPF_GRID_OFF 0
+#define SVPF_GRID_ON 1
+int svpf_set_grid_path(SVPFState* state, int mode);
 int svpf_get_path(const SVPFState* state);
 
 /* include/svpf.cuh:626-632 */

@@ -76,9 +76,10 @@ struct svpf_impl {
     float med_r2 = 0.0f;  // last median pairwise r^2 (a search seed, not a result)
     bool captured_coop = false;
     bool coop_graph_failed = false;
-    // grid path: filters too large for one workgroup's LDS
-    bool grid = false;
+    // grid path (workspace allocated for 2 <= n <= SVPF_GRID_MAX_PARTICLES)
+    int grid_mode = SVPF_GRID_AUTO;
     float* dgrid_ws = nullptr;
+    int captured_path = -1, captured_smax = -1;
 
     svpf_fparams* hp() { return reinterpret_cast<svpf_fparams*>(hblock + kBlockParams); }
     svpf_fstate* hs() { return reinterpret_cast<svpf_fstate*>(hblock + kBlockFstate); }
@@ -325,14 +326,37 @@ int check_supported(SVPFState* s) {
 
 // The cooperative multi-workgroup kernel serves one filter when the state asks
 // for it (use_persistent_kernel) and the features it implements suffice.
-bool use_coop(const SVPFState* s) {
+bool coop_applies(const SVPFState* s) {
     const svpf_impl* I = s->impl;
-    return !I->grid && s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop;
+    return s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop;
+}
+
+// Path of one filter's step.  Forced by svpf_set_grid_path, or by a filter above
+// the single-workgroup capacity (tile < 0).  Automatic choice, from per-tick
+// timings on MI355X (plain and full-feature filters): the cooperative kernel for
+// 256 <= n < 1024 where it applies, the grid path from n = 1024 up and for
+// 256 <= n < 1024 where the cooperative kernel does not apply, one workgroup
+// below n = 256.
+constexpr int kGridAutoMinN = 1024;
+constexpr int kCoopMinN = 256;
+bool use_grid(const SVPFState* s) {
+    const svpf_impl* I = s->impl;
+    if (!I->dgrid_ws) return false;
+    if (I->tile < 0 || I->grid_mode == SVPF_GRID_ON) return true;
+    if (I->grid_mode == SVPF_GRID_OFF) return false;
+    if (I->n >= kGridAutoMinN) return true;
+    return I->n >= kCoopMinN && !coop_applies(s);
+}
+
+bool use_coop(const SVPFState* s) { return !use_grid(s) && coop_applies(s); }
+
+int step_path(const SVPFState* s) {
+    return use_grid(s) ? SVPF_PATH_GRID : (use_coop(s) ? SVPF_PATH_COOP : SVPF_PATH_SINGLE);
 }
 
 hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
     svpf_impl* I = s->impl;
-    if (I->grid)
+    if (use_grid(s))
         return svpf_launch_grid(I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->d_lw(), I->dgrid_ws,
                                 I->n, s->stein_max_steps, t0, t1, cs);
     if (use_coop(s))
@@ -433,7 +457,10 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     hipStream_t cs = s->stream;
     hipError_t e = hipSuccess;
     const bool coop = use_coop(s);
-    if (I->captured && I->captured_coop != coop) svpf_graph_invalidate(s);  // path changed since capture
+    const int path = step_path(s);
+    // the captured launch sequence depends on the path and, on the grid path, on stein_max_steps
+    if (I->captured && (I->captured_path != path || (path == SVPF_PATH_GRID && I->captured_smax != s->stein_max_steps)))
+        svpf_graph_invalidate(s);
     if (use_graph && coop && I->coop_graph_failed) use_graph = false;
     if (use_graph) {
         if (!I->captured) {
@@ -450,6 +477,8 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
                     I->graph = g;
                     I->captured = true;
                     I->captured_coop = coop;
+                    I->captured_path = path;
+                    I->captured_smax = s->stein_max_steps;
                 } else if (g) {
                     hipGraphDestroy(g);
                 }
@@ -523,9 +552,11 @@ extern "C" {
 SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t stream) {
     if (n_particles < 1 || nu <= 0.0f) return nullptr;
     int tile = -1, bs = 256;
-    // one workgroup per filter while its LDS holds the filter, else the grid path
-    const bool grid = !choose_tile(n_particles, &tile, &bs);
-    if (grid && (n_particles > SVPF_GRID_MAX_PARTICLES || svpf_grid_init() != hipSuccess)) return nullptr;
+    // one workgroup per filter while its LDS holds the filter, else the grid path only
+    const bool grid_only = !choose_tile(n_particles, &tile, &bs);
+    if (n_particles > SVPF_GRID_MAX_PARTICLES) return nullptr;
+    const bool grid = n_particles >= 2;
+    if (grid && svpf_grid_init() != hipSuccess) return nullptr;
     if (svpf_kernels_init() != hipSuccess) return nullptr;
     SVPFState* s = static_cast<SVPFState*>(std::calloc(1, sizeof(SVPFState)));
     if (!s) return nullptr;
@@ -538,7 +569,6 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
     I->n = n_particles;
     I->tile = tile;
     I->bs = bs;
-    I->grid = grid;
     bool ok = hipMalloc(&I->dblock, kBlockBytes) == hipSuccess;
     if (grid) ok = ok && hipMalloc(&I->dgrid_ws, svpf_grid_workspace_floats(n_particles) * sizeof(float)) == hipSuccess;
     ok = ok && hipHostMalloc(&I->hblock, kBlockBytes, hipHostMallocDefault) == hipSuccess;
@@ -556,7 +586,7 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
         ok = ok && hipMemset(I->dblock, 0, kBlockBytes) == hipSuccess;
         ok = ok && hipMemset(I->xbuf, 0, xwords * sizeof(uint64_t)) == hipSuccess;
     }
-    if (ok && !grid) choose_coop_shape(I, -1, -1);
+    if (ok && !grid_only) choose_coop_shape(I, -1, -1);
     if (!ok) {
         svpf_destroy(s);
         return nullptr;
@@ -1000,7 +1030,7 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
 int svpf_set_coop_shape(SVPFState* s, int lanes_per_row, int xcd_stride) {
     if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
     svpf_impl* I = s->impl;
-    if (I->grid) return set_err(s, SVPF_ERR_UNSUPPORTED);
+    if (I->tile < 0) return set_err(s, SVPF_ERR_UNSUPPORTED);
     svpf_impl trial = {};
     trial.n = I->n;
     if (!choose_coop_shape(&trial, lanes_per_row, xcd_stride)) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
@@ -1012,33 +1042,20 @@ int svpf_set_coop_shape(SVPFState* s, int lanes_per_row, int xcd_stride) {
     return set_err(s, SVPF_OK);
 }
 
-int svpf_set_grid_path(SVPFState* s, int enable) {
+int svpf_set_grid_path(SVPFState* s, int mode) {
     if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
     svpf_impl* I = s->impl;
-    if (!enable) {
-        if (I->tile < 0) return set_err(s, SVPF_ERR_UNSUPPORTED);
-        if (I->grid) svpf_graph_invalidate(s);
-        I->grid = false;
-        return set_err(s, SVPF_OK);
-    }
-    if (I->n < 2 || I->n > SVPF_GRID_MAX_PARTICLES) return set_err(s, SVPF_ERR_UNSUPPORTED);
-    hipError_t e = svpf_grid_init();
-    if (e == hipSuccess && !I->dgrid_ws)
-        e = hipMalloc(&I->dgrid_ws, svpf_grid_workspace_floats(I->n) * sizeof(float));
-    if (e != hipSuccess) {
-        I->dgrid_ws = nullptr;
-        I->last_hip = e;
-        return set_err(s, e == hipErrorOutOfMemory ? SVPF_ERR_OUT_OF_MEMORY : SVPF_ERR_HIP);
-    }
-    if (!I->grid) svpf_graph_invalidate(s);
-    I->grid = true;
+    if (mode != SVPF_GRID_AUTO && mode != SVPF_GRID_OFF && mode != SVPF_GRID_ON)
+        return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+    if ((mode == SVPF_GRID_OFF && I->tile < 0) || (mode == SVPF_GRID_ON && !I->dgrid_ws))
+        return set_err(s, SVPF_ERR_UNSUPPORTED);
+    I->grid_mode = mode;
     return set_err(s, SVPF_OK);
 }
 
 int svpf_get_path(const SVPFState* s) {
     if (!s || !s->impl) return -1;
-    if (s->impl->grid) return SVPF_PATH_GRID;
-    return use_coop(s) ? SVPF_PATH_COOP : SVPF_PATH_SINGLE;
+    return step_path(s);
 }
 
 int svpf_get_coop_shape(const SVPFState* s, int* lanes_per_row, int* workgroups, int* xcd_stride) {

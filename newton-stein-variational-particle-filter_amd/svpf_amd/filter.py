@@ -23,10 +23,11 @@ class Filter:
     """A single SVPF filter driven through libsvpf_amd.so."""
 
     def __init__(self, cfg: dict, stream=None, path: str = "auto", coop_shape=None):
-        """path: "auto" (the library default: cooperative multi-workgroup kernel
-        for n >= 256, the grid path above the single-workgroup capacity),
-        "coop", "single" (one workgroup per filter) or "grid" (per-tick
-        sequence of grid-wide kernels, any n >= 2)."""
+        """path: "auto" (the library's choice by n and features: one workgroup
+        below n = 256, the cooperative multi-workgroup kernel for 256 <= n <
+        1024 where it applies, else the grid path), "single" (one workgroup),
+        "coop" (cooperative kernel when the features allow it, else one
+        workgroup) or "grid" (per-tick sequence of grid-wide kernels, n >= 2)."""
         self.L = _lib.load()
         self.cfg = dict(cfg)
         n = int(cfg["n"])
@@ -38,10 +39,12 @@ class Filter:
         self.params = _lib.SVPFParams(cfg["rho"], cfg["sigma_z"], cfg["mu"], cfg["gamma"])
         self._keep = []
         self.apply_config(cfg)
-        if path == "single":
-            self.state.use_persistent_kernel = 0
-        elif path == "coop":
-            self.state.use_persistent_kernel = 1
+        if path in ("single", "coop"):
+            rc = self.L.svpf_set_grid_path(self.sp, 0)
+            if rc:
+                raise RuntimeError(f"svpf_set_grid_path(n={n}, 0) rejected ({rc}): n is above the "
+                                   "single-workgroup capacity")
+            self.state.use_persistent_kernel = 1 if path == "coop" else 0
         elif path == "grid":
             rc = self.L.svpf_set_grid_path(self.sp, 1)
             if rc:
