@@ -206,6 +206,21 @@ def cpu_large_baseline(n=4096, ticks=2):
     return (time.perf_counter() - t0) / ticks
 
 
+def measured_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed PMC summary
+    (FETCH_SIZE and WRITE_SIZE runs of `bench.py --only dominant`, each scaled by
+    what the counter reads for a 256 MiB copy)."""
+    path = os.path.join(ROOT, "profiles", "svpf_batched_kernel_n2048_group_trace.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except (OSError, ValueError):
+        pass
+    return None
+
+
 def cpu_threads():
     """Host threads for the CPU baselines: the box's CPU share (at most 16)."""
     return max(1, min(16, os.cpu_count() or 1))
@@ -557,7 +572,10 @@ def main():
             "peak": PEAK_FP32 / 1e12,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32,
-            "traffic": None,
+            # the group's state is loaded and stored once per launch, whatever the tick count
+            "traffic": measured_traffic(),
+            "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the 5-tick launch, "
+                              "profiles/svpf_batched_kernel_n2048_group_trace.md",
             "kernel": f"svpf_batched_kernel<{int(groups['bs'][0])},{TILES[int(groups['tile'][0])][0]},"
                       f"{TILES[int(groups['tile'][0])][1]}> (N=2048 group, {nf_dom} filters, grid {int(groups['grid'][0])})",
             "kernel_ms": ms_dom,
