@@ -2,8 +2,8 @@
 //
 // Latency form of the tick for a single filter: workgroup g owns rows
 // [g*ROWS, (g+1)*ROWS) (ROWS = BS/CS, one row per CS adjacent lanes).  Work
-// that is O(N) -- predict, guide, the median bandwidth, the sorted mixture
-// means -- is done redundantly by every workgroup from identical inputs, so
+// that is O(N) -- predict, guide, the median bandwidth, the mixture means --
+// is done redundantly by every workgroup from identical inputs, so
 // the only cross-workgroup traffic per tick is
 //   * one exchange per Stein iteration of the transport records
 //     {h/bw, s, c/bw} of every particle, and
@@ -44,7 +44,7 @@ struct CoopEngine {
 
     // LDS
     float4* hsc;     // [n]  transport records of all particles
-    float* mbuf;     // [p2] sorted scaled mixture means / median sort buffer
+    float* mbuf;     // [p2] scaled mixture means / median sort buffer
     float* hall;     // [n]  positions of all particles (predicted, then guided)
     float* hfin;     // [n]  final positions of the previous tick (this tick's h_prev)
     float* hpp;      // [n]  final positions two ticks back (MIM asymmetric rho)
@@ -248,7 +248,8 @@ struct CoopEngine {
             const float ib = sc->ib, rs = sc->rs;
             SVPF_TRACE_POINT(t, 3);
 
-            // ---- scaled mixture means ----
+            // ---- scaled mixture means (unsorted: at this size a sort per tick costs more than
+            //      the nearest-component pass it would replace) ----
             const float inv_s2 = 1.0f / (sigma * sigma);
             const float sq = sqrtf(1.4426950408889634f * 0.5f * inv_s2);
             {
@@ -285,7 +286,7 @@ struct CoopEngine {
                     // gradient of the own row (src/svpf_opt_kernels.cu:354-521)
                     float sr[1], srd[1];
                     E.h[0] = h;
-                    E.prior_sums(sq, sr, srd);
+                    E.template prior_sums<false>(sq, sr, srd);
                     float sj, cj, cv;
                     row_gradient(P, beta, y_t, inv_s2, sq, sgn2, mode, sr[0], srd[0], sj, cj, cv);
                     SVPF_TRACE_POINT(t, 8 + 3 * it_row);

@@ -130,18 +130,65 @@ struct Engine {
     // r_i = exp(-(h_j-m_i)^2/(2 sigma^2) - max)  (:453-470).
     //
     // Evaluated in units scaled by sq = sqrt(log2(e) / (2 sigma^2)): with
-    // hh = sq h and mm = sq m (mbuf holds the mm, unsorted, padded with
-    // far-away sentinels to a multiple of 4), r_i = exp2(dd_min^2 - (hh - mm_i)^2).
-    // As in the reference, a first pass finds the shift dd_min^2 (the nearest
-    // component, the reference's max term) and a second pass accumulates:
-    // a pair costs sub + mul + min, then sub + fma + exp2 + two accumulates.
+    // hh = sq h and mm = sq m, r_i = exp2(dd_min^2 - (hh - mm_i)^2).  The
+    // means depend only on the previous tick's particles, so mbuf holds them
+    // sorted ascending for the whole Stein loop (sort_means), padded with
+    // far-away sentinels to a multiple of 4.  The shift dd_min^2 (the nearest
+    // component, the reference's max term) is then a binary search per row
+    // (nearest_sq); one pass over the means accumulates: a pair costs
+    // sub + fma + exp2 + two accumulates.
     // ---------------------------------------------------------------------
+
+    // Scaled means of the own rows -> mbuf, sorted ascending, sentinels in [n, p2).
+    // The transport records are not live between the bandwidth and the first
+    // gradient of a tick: their LDS is the sort input.  All threads call it.
+    template <class MeanOf>
+    SVPF_DEV void sort_means(MeanOf mean_of) {
+        float* tmp = reinterpret_cast<float*>(hsc);
+        if (cs == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                int i = row(r);
+                if (i < n) tmp[i] = mean_of(r);
+            }
+        }
+        __syncthreads();
+        block_sort<BS>(tmp, mbuf, n);
+        for (int i = n + threadIdx.x; i < p2; i += BS) mbuf[i] = 1.0e18f;  // exp2 underflows to 0
+        __syncthreads();
+    }
+
+    // min_i fl(hh - mm_i)^2 over the sorted means: rounding is monotone, so it
+    // is attained at the largest mean below hh or the smallest one at or above
+    // it -- bit-identical to the minimum over all means.
+    SVPF_DEV float nearest_sq(float hh) const {
+        int pos = -1;  // last index with mbuf[pos] < hh
+        for (int step = 1 << (31 - __clz(n)); step > 0; step >>= 1) {
+            const int q = pos + step;
+            pos = (q < n && mbuf[q] < hh) ? q : pos;
+        }
+        float best = 3.0e38f;
+        if (pos >= 0) {
+            const float d = hh - mbuf[pos];
+            best = d * d;
+        }
+        if (pos + 1 < n) {
+            const float d = hh - mbuf[pos + 1];
+            best = fminf(best, d * d);
+        }
+        return best;
+    }
+
+    // SORTED: mbuf holds the means sorted (sort_means) and the shift is a binary
+    // search; otherwise (unsorted means) a first pass over the means finds it:
+    // sub + mul + min per pair.
+    template <bool SORTED = true>
     SVPF_DEV void prior_sums(float sq, float (&sr)[R], float (&srd)[R]) const {
         float hh[R], off[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             hh[r] = h[r] * sq;
-            off[r] = 3.0e38f;
+            off[r] = SORTED ? nearest_sq(hh[r]) : 3.0e38f;
             sr[r] = 0.f;
             srd[r] = 0.f;
         }
@@ -149,13 +196,26 @@ struct Engine {
         const float4* m4 = reinterpret_cast<const float4*>(mbuf);
         // U loads in flight per step: at one wave per SIMD nothing else hides the LDS latency
         constexpr int U = R >= 2 ? 2 : 4;
-        auto pass1 = [&](const float4& m) {
+        if constexpr (!SORTED) {
+            auto pass1 = [&](const float4& m) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) {
-                float d0 = hh[r] - m.x, d1 = hh[r] - m.y, d2 = hh[r] - m.z, d3 = hh[r] - m.w;
-                off[r] = fminf(off[r], fminf(fminf(d0 * d0, d1 * d1), fminf(d2 * d2, d3 * d3)));
+                for (int r = 0; r < R; ++r) {
+                    float d0 = hh[r] - m.x, d1 = hh[r] - m.y, d2 = hh[r] - m.z, d3 = hh[r] - m.w;
+                    off[r] = fminf(off[r], fminf(fminf(d0 * d0, d1 * d1), fminf(d2 * d2, d3 * d3)));
+                }
+            };
+            int q = cs;
+            for (; q + (U - 1) * CS < nq; q += U * CS) {
+                float4 m[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) m[u] = m4[q + u * CS];
+#pragma unroll
+                for (int u = 0; u < U; ++u) pass1(m[u]);
             }
-        };
+            for (; q < nq; q += CS) pass1(m4[q]);
+#pragma unroll
+            for (int r = 0; r < R; ++r) off[r] = group_min<CS>(off[r]);
+        }
         auto pass2 = [&](const float4& m) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -169,17 +229,6 @@ struct Engine {
             }
         };
         int q = cs;
-        for (; q + (U - 1) * CS < nq; q += U * CS) {
-            float4 m[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) m[u] = m4[q + u * CS];
-#pragma unroll
-            for (int u = 0; u < U; ++u) pass1(m[u]);
-        }
-        for (; q < nq; q += CS) pass1(m4[q]);
-#pragma unroll
-        for (int r = 0; r < R; ++r) off[r] = group_min<CS>(off[r]);
-        q = cs;
         for (; q + (U - 1) * CS < nq; q += U * CS) {
             float4 m[U];
 #pragma unroll
@@ -429,15 +478,7 @@ struct Engine {
         const float sq = sqrtf(1.4426950408889634f * 0.5f * inv_s2);
         {
             const float lev_num = P.gamma * y_prev;
-            if (cs == 0) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    int i = row(r);
-                    if (i < n) mbuf[i] = sq * (mu + P.rho * (hp[r] - mu) + lev_num / (expf(hp[r] * 0.5f) + 1e-8f));
-                }
-            }
-            for (int i = n + threadIdx.x; i < p2; i += BS) mbuf[i] = 1.0e18f;  // sentinels: exp2 underflows to 0
-            __syncthreads();
+            sort_means([&](int r) { return sq * (mu + P.rho * (hp[r] - mu) + lev_num / (expf(hp[r] * 0.5f) + 1e-8f)); });
         }
 
         // ---- annealed Stein loop (src/svpf_optimized_graph.cu:1057-1317) ----

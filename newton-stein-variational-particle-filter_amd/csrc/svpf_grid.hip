@@ -8,8 +8,13 @@
 //   grid_prologue    1 workgroup   scalar prologue: adaptive mu / sigma, EKF guide, Stein budget
 //   grid_predict     n/256         predict (plain / MIM / guided), predictive log weights, guide,
 //                                  block partials for the reductions
-//   grid_bandwidth   1 x 1024      predictive loglik; bandwidth (median pairwise r^2 by an LDS sort and
-//                                  the exact selection with 16-bit row brackets, or the EMA / Silverman rules)
+//   grid_bandwidth   1 workgroup   predictive loglik; EMA / Silverman bandwidth
+//   grid_sort_runs   n/1024        median rule: sorted runs of 1024 particles (LDS)
+//   grid_sort_merge  n/256         rank of every particle across the runs -> sorted particles
+//   grid_median_round x 2..5  (n-1)/256  median pairwise r^2, one round each: apply the previous round's
+//                                  counts to every row's bracket, count pairs at or below 15 thresholds;
+//                                  the converging round gathers the candidates
+//   grid_median_final 1 x 1024     rounds still needed if any, rank of the candidates -> bandwidth
 //   grid_means       n/256         scaled mixture-component means
 //   per Stein iteration k < max:
 //     grid_gradient  G x 256       mixture prior (component means streamed through LDS tiles) + likelihood
@@ -25,6 +30,8 @@
 // graph).  Rows of the O(N^2) kernels are spread CS lanes per row so that G
 // covers the CUs; the per-particle math is the engine's (svpf_engine.hpp).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "svpf_engine.hpp"
 #include "svpf_kernels.h"
@@ -65,6 +72,159 @@ SVPF_DEV void iter_schedule(const svpf_fparams& P, const svpf_fstate& S, int bud
     else if (F & SVPF_F_LINEAR_BETA) beta = na > 1 ? 0.3f + 0.7f * (float)ai / (float)(na - 1) : 1.0f;
     else if (F & SVPF_F_ADAPT_BETA) beta = Engine<256, 1, 1>::adaptive_beta(S.ksd_prev, ai, na);
     else beta = (ai % 3 == 0) ? 0.3f : ((ai % 3 == 1) ? 0.65f : 1.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Median of the pairwise r^2 over the grid.  The same selection as the
+// single-workgroup median_pairwise_r2 (svpf_device.hpp): bisection over the
+// bit patterns of non-negative floats with row brackets [plo, phi) of the pairs
+// still inside [L, H], the first round seeded around the previous tick's value,
+// extended downwards / upwards 8x wider while the rank lies outside, then even
+// splits; here with 15 thresholds a round, one round per launch over (n-1)/256
+// workgroups of one row per lane, and up to 4096 candidates gathered at the end
+// and sorted.  Counts are exact integers, so the result is the float32 value of
+// rank floor((M-1)/2) among the M = n(n-1)/2 values fl(fl(s_j - s_i)^2).
+// ---------------------------------------------------------------------------
+constexpr int kMW = 15;            // thresholds per round
+constexpr int kMedCand = 1024;     // candidates gathered for the final selection
+constexpr int kMaxMedRounds = 5;   // round launches per tick (med_rounds); rare further rounds run in grid_median_final
+constexpr int kMedRows = 256;      // rows per round workgroup
+constexpr int kRun = 1024;         // sorted run length of the two-kernel sort
+constexpr int kMaxMG = (kMaxN + kMedRows - 1) / kMedRows;
+constexpr uint32_t kSeedStep = 1u << 19;  // 1/16 octave of r^2 in float bit patterns
+enum { MS_SEED = 0, MS_DOWN = 1, MS_UP = 2, MS_SPLIT = 3 };
+
+struct MedState {
+    uint32_t L, H, cL, cH;  // bracket [L, H] of bit patterns; pairs below L, pairs at or below H
+    int64_t anchor, step;
+    int mode, fresh, done, gathered;  // fresh: thresholds not counted yet; gathered: candidates in mcand
+};
+
+// round launches per tick for n particles: the rounds a seeded selection needs at that size
+int med_rounds(int n) { return n <= 1024 ? 2 : (n <= 4096 ? 3 : kMaxMedRounds); }
+
+// dst[0, cnt) = src[0, cnt) with 16-byte loads, four in flight per thread (src, dst 16-byte aligned)
+template <int BS>
+SVPF_DEV void lds_fill(float* __restrict__ dst, const float* __restrict__ src, int cnt) {
+    const int n4 = cnt >> 2;
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    int k = threadIdx.x;
+    for (; k + 3 * BS < n4; k += 4 * BS) {
+        const float4 a = s4[k], b = s4[k + BS], c = s4[k + 2 * BS], d = s4[k + 3 * BS];
+        d4[k] = a;
+        d4[k + BS] = b;
+        d4[k + 2 * BS] = c;
+        d4[k + 3 * BS] = d;
+    }
+    for (; k < n4; k += BS) d4[k] = s4[k];
+    for (int i = (n4 << 2) + threadIdx.x; i < cnt; i += BS) dst[i] = src[i];
+}
+
+// initial selection state from the sorted particles s[0, n) and the previous tick's value
+SVPF_DEV MedState med_init(const float* s, int n, float est) {
+    MedState st;
+    const float dmax = s[n - 1] - s[0];
+    st.L = 0u;
+    st.H = __float_as_uint(dmax * dmax);
+    st.cL = 0u;
+    st.cH = (uint32_t)(((uint64_t)n * (uint64_t)(n - 1)) / 2);
+    if (!(est > 0.0f && est < 3.0e38f)) {
+        const float iq = s[(3 * n) / 4] - s[n / 4];
+        est = 0.5f * iq * iq;
+    }
+    const uint32_t eb = (est > 0.0f && est < 3.0e38f) ? __float_as_uint(est) : 0u;
+    st.mode = (eb > st.L && eb < st.H) ? MS_SEED : MS_SPLIT;
+    st.anchor = (int64_t)eb;
+    st.step = (int64_t)kSeedStep;
+    st.fresh = 1;
+    st.done = (!(st.cH - st.cL > (uint32_t)kMedCand && st.L < st.H)) ? 1 : 0;
+    st.gathered = 0;
+    return st;
+}
+
+SVPF_DEV uint32_t med_need(int n) {
+    const uint32_t M = (uint32_t)(((uint64_t)n * (uint64_t)(n - 1)) / 2);
+    return (M - 1u) / 2u + 1u;  // count(<= x) must reach rank + 1
+}
+
+// thresholds of a state, non-decreasing in q, inside [L, H)
+SVPF_DEV uint32_t med_thr(const MedState& st, int q) {
+    int64_t x;
+    if (st.mode == MS_SEED) x = st.anchor + (int64_t)(q - kMW / 2) * st.step;
+    else if (st.mode == MS_DOWN) x = st.anchor - (int64_t)(kMW - q) * st.step;
+    else if (st.mode == MS_UP) x = st.anchor + (int64_t)(q + 1) * st.step;
+    else return st.L + (uint32_t)(((uint64_t)(st.H - st.L) * (uint64_t)(q + 1)) / (kMW + 1));
+    x = x < (int64_t)st.L ? (int64_t)st.L : x;
+    x = x > (int64_t)st.H - 1 ? (int64_t)st.H - 1 : x;
+    return (uint32_t)x;
+}
+
+SVPF_DEV bool med_converged(const MedState& st) { return !(st.cH - st.cL > (uint32_t)kMedCand && st.L < st.H); }
+
+// Applies the totals c[q] (pairs at or below thr(q)) of the state's thresholds:
+// the sub-bracket holding the rank, the next mode.  qs / xlo / xhi drive the
+// row bracket updates (med_row_update).
+SVPF_DEV void med_advance(MedState& st, const uint32_t* c, uint32_t need, int& qs, uint32_t& xlo, uint32_t& xhi) {
+    qs = kMW;
+    for (int q = kMW - 1; q >= 0; --q)
+        if (c[q] >= need) qs = q;
+    xlo = qs > 0 ? med_thr(st, qs - 1) : 0u;
+    xhi = qs < kMW ? med_thr(st, qs) : 0u;
+    const uint32_t clo = qs > 0 ? c[qs - 1] : 0u, chi = qs < kMW ? c[qs] : 0u;
+    if ((st.mode == MS_SEED || st.mode == MS_DOWN) && qs == 0) {
+        st.anchor = (int64_t)med_thr(st, 0);  // still below the grid: extend it downwards
+        st.step *= 8;
+        st.mode = MS_DOWN;
+    } else if ((st.mode == MS_SEED || st.mode == MS_UP) && qs == kMW) {
+        st.anchor = (int64_t)med_thr(st, kMW - 1);
+        st.step *= 8;
+        st.mode = MS_UP;
+    } else {
+        st.mode = MS_SPLIT;
+    }
+    if (qs > 0) {
+        st.L = xlo + 1u;
+        st.cL = clo;
+    }
+    if (qs < kMW) {
+        st.H = xhi;
+        st.cH = chi;
+    }
+    st.fresh = 0;
+    st.done = med_converged(st) ? 1 : 0;
+}
+
+SVPF_DEV void med_row_update(const float* s, int i, int qs, uint32_t xlo, uint32_t xhi, int* plo, int* phi) {
+    const float si = s[i];
+    int a = plo[i], b = phi[i];
+    if (qs > 0) a = row_first_gt(s, si, a, b, __uint_as_float(xlo));
+    if (qs < kMW) b = row_first_gt(s, si, a, b, __uint_as_float(xhi));
+    plo[i] = a;
+    phi[i] = b;
+}
+
+// counts of row i at or below the thresholds xb (c[q] += pairs (i, j) with fl(fl(s_j - s_i)^2) <= xb[q])
+SVPF_DEV void med_row_count(const float* s, int n, int i, const int* plo, const int* phi, const uint32_t (&xb)[kMW],
+                            uint32_t (&c)[kMW]) {
+    int res[kMW];
+    row_search<kMW>(s, n, s[i], plo[i], phi[i], xb, res);
+#pragma unroll
+    for (int q = 0; q < kMW; ++q) c[q] += (uint32_t)(res[q] - (i + 1));
+}
+
+// workgroup totals of c[kMW] into tot[kMW] (LDS), every thread reads them after the call
+template <int BS>
+SVPF_DEV void med_block_totals(uint32_t (&c)[kMW], uint32_t* tot) {
+    if (threadIdx.x < kMW) tot[threadIdx.x] = 0u;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kMW; ++q) c[q] = wave_sum_u32(c[q]);
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int q = 0; q < kMW; ++q) atomicAdd(&tot[q], c[q]);
+    }
+    __syncthreads();
 }
 
 __global__ void grid_prologue(const svpf_fparams* __restrict__ Pp, svpf_fstate* __restrict__ Sp,
@@ -226,9 +386,7 @@ __global__ __launch_bounds__(kGB) void grid_predict(const svpf_fparams* __restri
 // sort of n floats and the exact median selection with 16-bit row brackets
 __global__ __launch_bounds__(1024) void grid_bandwidth(const svpf_fparams* __restrict__ Pp,
                                                        svpf_fstate* __restrict__ Sp, GridScalars* __restrict__ g,
-                                                       const float* __restrict__ h, const float* __restrict__ part,
-                                                       int nb, int n) {
-    extern __shared__ __align__(16) char lds[];
+                                                       const float* __restrict__ part, int nb, int n) {
     constexpr int BS = 1024;
     __shared__ float red[80];
     const svpf_fparams& P = *Pp;
@@ -253,20 +411,7 @@ __global__ __launch_bounds__(1024) void grid_bandwidth(const svpf_fparams* __res
     const float inv_n = 1.0f / (float)n;
     float bw, bsq;
     if (P.bandwidth_mode == SVPF_BW_MEDIAN) {
-        int ps = 128;
-        while (ps < n) ps <<= 1;
-        float* src = reinterpret_cast<float*>(lds);            // ps floats; then the 16-bit brackets
-        float* dst = src + ps;                                  // ps + 4 floats (sorted, +inf sentinel)
-        float* cand = dst + ps + 4;                             // kMedianCands floats
-        for (int i = threadIdx.x; i < n; i += BS) src[i] = h[i];
-        __syncthreads();
-        block_sort<BS>(src, dst, n);
-        uint16_t* plo = reinterpret_cast<uint16_t*>(src);
-        const float med = median_pairwise_r2<BS, 2, uint16_t>(dst, n, S.med_r2, plo, plo + n, cand,
-                                                              min(kMedianCands, 2 * n), reinterpret_cast<uint32_t*>(red));
-        bw = fminf(fmaxf(sqrtf(med), 0.01f), 2.0f);
-        bsq = med;
-        if (threadIdx.x == 0) S.med_r2 = med;
+        bw = bsq = 0.0f;  // grid_median_final writes the bandwidth
     } else {
         const float mean = sums[1] * inv_n, var = sums[2] * inv_n - mean * mean;
         if (P.bandwidth_mode == SVPF_BW_SILVERMAN) {
@@ -296,8 +441,253 @@ __global__ __launch_bounds__(1024) void grid_bandwidth(const svpf_fparams* __res
     }
     if (threadIdx.x == 0) {
         g->pred_ll = M + logf(fmaxf(sums[0] / (float)n, 1e-10f));
+        if (P.bandwidth_mode == SVPF_BW_MEDIAN) return;
         g->bw = bw;
         g->bw_sq = bsq;
+        g->ib = 1.0f / (bw * bw);
+        g->rs = 1.0f / bw;
+    }
+}
+
+// Sort of the predicted particles for the median rule, in two launches.
+// grid_sort_runs: workgroup b sorts h[b*1024, b*1024 + 1024) in LDS into runs.
+__global__ __launch_bounds__(kRun) void grid_sort_runs(const svpf_fparams* __restrict__ Pp,
+                                                       const float* __restrict__ h, float* __restrict__ runs, int n) {
+    __shared__ __align__(16) float src[kRun], dst[kRun];
+    if (Pp->bandwidth_mode != SVPF_BW_MEDIAN) return;
+    const int base = blockIdx.x * kRun, len = min(kRun, n - base);
+    if (threadIdx.x < len) src[threadIdx.x] = h[base + threadIdx.x];
+    __syncthreads();
+    block_sort<kRun>(src, dst, len);
+    if (threadIdx.x < len) runs[base + threadIdx.x] = dst[threadIdx.x];
+}
+
+// grid_sort_merge: one element per lane; its rank in the merged order is its
+// index in its own run plus, in every other run, the elements before it in a
+// stable merge (<= x in lower runs, < x in higher runs), so ranks are distinct.
+__global__ __launch_bounds__(kGB) void grid_sort_merge(const svpf_fparams* __restrict__ Pp,
+                                                       const float* __restrict__ runs, float* __restrict__ srt, int n) {
+    extern __shared__ __align__(16) char lds[];
+    if (Pp->bandwidth_mode != SVPF_BW_MEDIAN) return;
+    float* r = reinterpret_cast<float*>(lds);  // n floats: all runs
+    lds_fill<kGB>(r, runs, n);
+    __syncthreads();
+    const int i = blockIdx.x * kGB + threadIdx.x;
+    if (i == 0) srt[n] = 3.402823466e38f;
+    if (i >= n) return;
+    const int a = i / kRun, nr = (n + kRun - 1) / kRun;
+    const float x = r[i];
+    int rank = i - a * kRun;
+    for (int b = 0; b < nr; ++b) {
+        if (b == a) continue;
+        const int lo = b * kRun, len = min(kRun, n - lo);
+        // elements of run b before x: y <= x (b < a) or y < x (b > a)
+        int pos = 0;
+        for (int step = kRun; step > 0; step >>= 1) {
+            const int q = pos + step;
+            if (q <= len) {
+                const float y = r[lo + q - 1];
+                pos = (b < a ? (y <= x) : (y < x)) ? q : pos;
+            }
+        }
+        rank += pos;
+    }
+    srt[rank] = x;
+}
+
+// One selection round r of med_rounds(n): workgroup b owns rows [b*256, b*256+256).
+// Round 0 builds the initial state and brackets; later rounds first apply the
+// previous round's counts.  The round that reaches the final bracket gathers
+// its rows' candidates into mcand.
+__global__ __launch_bounds__(kMedRows) void grid_median_round(const svpf_fparams* __restrict__ Pp,
+                                                              const svpf_fstate* __restrict__ Sp,
+                                                              const float* __restrict__ srt, int* __restrict__ plo,
+                                                              int* __restrict__ phi, MedState* __restrict__ mstate,
+                                                              uint32_t* __restrict__ mpart, float* __restrict__ mcand,
+                                                              uint32_t* __restrict__ gcount, int r, int n) {
+    extern __shared__ __align__(16) char lds[];
+    __shared__ uint32_t tot[kMW];
+    __shared__ uint32_t wsum[kMedRows / kWave];
+    __shared__ uint32_t cbase;
+    if (Pp->bandwidth_mode != SVPF_BW_MEDIAN) return;
+    MedState st;
+    if (r > 0) {
+        st = mstate[(r + 1) & 1];
+        if (st.done) return;  // uniform
+    }
+    float* s = reinterpret_cast<float*>(lds);  // n + 1 floats: sorted particles, +inf at n
+    lds_fill<kMedRows>(s, srt, n + 1);
+    const int rows = n - 1;
+    const int nmg = (rows + kMedRows - 1) / kMedRows;
+    const int i = blockIdx.x * kMedRows + threadIdx.x;
+    if (r > 0 && threadIdx.x < kMW) {
+        // totals of the previous round, summed in a fixed order (identical in every workgroup)
+        const uint32_t* pp = mpart + (size_t)((r + 1) & 1) * kMaxMG * kMW;
+        uint32_t acc = 0u;
+        for (int b = 0; b < nmg; ++b) acc += pp[b * kMW + threadIdx.x];
+        tot[threadIdx.x] = acc;
+    }
+    __syncthreads();
+    if (r == 0) {
+        st = med_init(s, n, Sp->med_r2);
+        if (i < rows) {
+            plo[i] = i + 1;
+            phi[i] = n;
+        }
+        st.fresh = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            *gcount = 0u;
+            mstate[1].done = 0;  // no stale result from the previous tick
+            mstate[0] = st;
+        }
+        if (st.done) return;  // uniform: grid_median_final gathers
+    } else {
+        int qs;
+        uint32_t xlo, xhi;
+        med_advance(st, tot, med_need(n), qs, xlo, xhi);
+        if (i < rows) med_row_update(s, i, qs, xlo, xhi, plo, phi);
+        if (st.done) {
+            if (st.L < st.H) {
+                // gather this workgroup's candidates at an offset reserved from gcount
+                const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+                const uint32_t cnt = i < rows ? (uint32_t)(phi[i] - plo[i]) : 0u;
+                const uint32_t incl = wave_scan_u32(cnt);
+                if (lane == kWave - 1) wsum[w] = incl;
+                __syncthreads();
+                uint32_t off = incl - cnt, all = 0u;
+                for (int q = 0; q < kMedRows / kWave; ++q) {
+                    off += (q < w) ? wsum[q] : 0u;
+                    all += wsum[q];
+                }
+                if (threadIdx.x == 0) cbase = all ? atomicAdd(gcount, all) : 0u;
+                __syncthreads();
+                off += cbase;
+                if (i < rows) {
+                    const float si = s[i];
+                    for (int j = plo[i]; j < phi[i]; ++j) {
+                        const float d = s[j] - si;
+                        mcand[off++] = d * d;
+                    }
+                }
+                st.gathered = 1;
+            }
+            if (blockIdx.x == 0 && threadIdx.x == 0) mstate[r & 1] = st;
+            return;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) mstate[r & 1] = st;
+    }
+    uint32_t xb[kMW], c[kMW];
+#pragma unroll
+    for (int q = 0; q < kMW; ++q) {
+        xb[q] = med_thr(st, q);
+        c[q] = 0u;
+    }
+    if (i < rows) med_row_count(s, n, i, plo, phi, xb, c);
+    __syncthreads();  // tot was read above
+    med_block_totals<kMedRows>(c, tot);
+    if (threadIdx.x < kMW) mpart[(size_t)(r & 1) * kMaxMG * kMW + blockIdx.x * kMW + threadIdx.x] = tot[threadIdx.x];
+}
+
+// Final selection: rounds still needed (none in the common case) run here over
+// all rows; the candidates (gathered by the converging round, or here) are
+// ranked by counting.  Writes the bandwidth.
+__global__ __launch_bounds__(1024) void grid_median_final(const svpf_fparams* __restrict__ Pp,
+                                                          svpf_fstate* __restrict__ Sp, GridScalars* __restrict__ g,
+                                                          const float* __restrict__ srt, int* __restrict__ plo,
+                                                          int* __restrict__ phi, const MedState* __restrict__ mstate,
+                                                          const uint32_t* __restrict__ mpart,
+                                                          const float* __restrict__ mcand, int rounds, int n) {
+    constexpr int BS = 1024;
+    extern __shared__ __align__(16) char lds[];
+    __shared__ uint32_t tot[kMW];
+    __shared__ uint32_t wsum[BS / kWave];
+    __shared__ float medv;
+    if (Pp->bandwidth_mode != SVPF_BW_MEDIAN) return;
+    float* s = reinterpret_cast<float*>(lds);  // n + 1 floats
+    float* cand = s + ((n + 4) & ~3);          // kMedCand floats
+    const int rows = n - 1;
+    const uint32_t need = med_need(n);
+    MedState st;
+    if (mstate[0].done) st = mstate[0];
+    else if (mstate[1].done) st = mstate[1];
+    else st = mstate[(rounds - 1) & 1];  // the last round's state, counted but not yet applied
+    if (!st.gathered) lds_fill<BS>(s, srt, n + 1);
+    __syncthreads();
+    if (!st.done) {
+        const uint32_t* pp = mpart + (size_t)((rounds - 1) & 1) * kMaxMG * kMW;
+        const int nmg = (rows + kMedRows - 1) / kMedRows;
+        if (threadIdx.x < kMW) {
+            uint32_t acc = 0u;
+            for (int b = 0; b < nmg; ++b) acc += pp[b * kMW + threadIdx.x];
+            tot[threadIdx.x] = acc;
+        }
+        __syncthreads();
+        int qs;
+        uint32_t xlo, xhi;
+        med_advance(st, tot, need, qs, xlo, xhi);
+        for (int i = threadIdx.x; i < rows; i += BS) med_row_update(s, i, qs, xlo, xhi, plo, phi);
+        __syncthreads();
+        while (!st.done) {  // uniform: every thread holds the same state
+            uint32_t xb[kMW], c[kMW];
+#pragma unroll
+            for (int q = 0; q < kMW; ++q) {
+                xb[q] = med_thr(st, q);
+                c[q] = 0u;
+            }
+            for (int i = threadIdx.x; i < rows; i += BS) med_row_count(s, n, i, plo, phi, xb, c);
+            med_block_totals<BS>(c, tot);
+            med_advance(st, tot, need, qs, xlo, xhi);
+            for (int i = threadIdx.x; i < rows; i += BS) med_row_update(s, i, qs, xlo, xhi, plo, phi);
+            __syncthreads();
+        }
+    }
+    float med;
+    if (st.L >= st.H) {
+        med = __uint_as_float(st.H);
+    } else {
+        const int C = (int)(st.cH - st.cL);
+        if (st.gathered) {
+            for (int k = threadIdx.x; k < C; k += BS) cand[k] = mcand[k];
+        } else {
+            // gather the candidates (pairs with r^2 in [L, H]) of every row
+            const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+            uint32_t cnt = 0u;
+            for (int i = threadIdx.x; i < rows; i += BS) cnt += (uint32_t)(phi[i] - plo[i]);
+            const uint32_t incl = wave_scan_u32(cnt);
+            if (lane == kWave - 1) wsum[w] = incl;
+            __syncthreads();
+            uint32_t off = incl - cnt;
+            for (int q = 0; q < w; ++q) off += wsum[q];
+            for (int i = threadIdx.x; i < rows; i += BS) {
+                const float si = s[i];
+                const int b = phi[i];
+                for (int j = plo[i]; j < b; ++j) {
+                    const float d = s[j] - si;
+                    cand[off++] = d * d;
+                }
+            }
+        }
+        __syncthreads();
+        // the candidate of rank need - cL - 1 among the C: below <= rank < at-or-below
+        const uint32_t r0 = need - st.cL - 1u;
+        for (int k = threadIdx.x; k < C; k += BS) {
+            const float v = cand[k];
+            uint32_t below = 0u, upto = 0u;
+            for (int m = 0; m < C; ++m) {
+                const float u = cand[m];
+                below += (u < v) ? 1u : 0u;
+                upto += (u <= v) ? 1u : 0u;
+            }
+            if (below <= r0 && r0 < upto) medv = v;
+        }
+        __syncthreads();
+        med = medv;
+    }
+    if (threadIdx.x == 0) {
+        const float bw = fminf(fmaxf(sqrtf(med), 0.01f), 2.0f);
+        Sp->med_r2 = med;
+        g->bw = bw;
+        g->bw_sq = med;
         g->ib = 1.0f / (bw * bw);
         g->rs = 1.0f / bw;
     }
@@ -624,38 +1014,107 @@ int grid_cs(int n) {
     return cs;
 }
 
+
+// Workspace of one filter (offsets in floats, each a multiple of 4):
+//   rec [n] float4 | mm [npad] | part [(nb + kMaxG) x GP_N] | scalars [32] | srt [n + 4] | runs [n] |
+//   plo [n] int | phi [n] int | mstate [2] MedState + gcount (32 floats) | mpart [2][kMaxMG][kMW] uint32 |
+//   mcand [kMedCand]
+struct GridWs {
+    float4* rec;
+    float* mm;
+    float* part;
+    GridScalars* gsc;
+    float* srt;
+    float* runs;
+    int* plo;
+    int* phi;
+    MedState* mstate;
+    uint32_t* gcount;
+    uint32_t* mpart;
+    float* mcand;
+};
+
+size_t grid_ws_layout(int n, float* base, GridWs* w) {
+    auto r4 = [](size_t x) { return (x + 3) & ~(size_t)3; };
+    const size_t nb = ((size_t)n + kGB - 1) / kGB;
+    size_t o = 0;
+    const size_t o_rec = o;
+    o += 4 * (size_t)n;
+    const size_t o_mm = o;
+    o += r4(n);
+    const size_t o_part = o;
+    o += r4((nb + kMaxG) * GP_N);
+    const size_t o_gsc = o;
+    o += 32;
+    const size_t o_srt = o;
+    o += r4((size_t)n + 4);
+    const size_t o_runs = o;
+    o += r4(n);
+    const size_t o_plo = o;
+    o += r4(n);
+    const size_t o_phi = o;
+    o += r4(n);
+    const size_t o_ms = o;
+    o += 32;
+    const size_t o_mp = o;
+    o += r4(2 * (size_t)kMaxMG * kMW);
+    const size_t o_mc = o;
+    o += kMedCand;
+    static_assert(sizeof(GridScalars) <= 32 * sizeof(float) && 2 * sizeof(MedState) <= 28 * sizeof(float), "ws slots");
+    if (w) {
+        w->rec = reinterpret_cast<float4*>(base + o_rec);
+        w->mm = base + o_mm;
+        w->part = base + o_part;
+        w->gsc = reinterpret_cast<GridScalars*>(base + o_gsc);
+        w->srt = base + o_srt;
+        w->runs = base + o_runs;
+        w->plo = reinterpret_cast<int*>(base + o_plo);
+        w->phi = reinterpret_cast<int*>(base + o_phi);
+        w->mstate = reinterpret_cast<MedState*>(base + o_ms);
+        w->gcount = reinterpret_cast<uint32_t*>(base + o_ms + 28);
+        w->mpart = reinterpret_cast<uint32_t*>(base + o_mp);
+        w->mcand = base + o_mc;
+    }
+    return o;
+}
+
+// dynamic LDS of the merge / round / final kernels
+size_t merge_lds_bytes(int n) { return (size_t)n * sizeof(float); }
+size_t round_lds_bytes(int n) { return ((size_t)n + 1) * sizeof(float); }
+size_t final_lds_bytes(int n) { return ((((size_t)n + 4) & ~(size_t)3) + (size_t)kMedCand) * sizeof(float); }
+
 }  // namespace
 
-extern "C" size_t svpf_grid_workspace_floats(int n) {
-    const int nb = (n + kGB - 1) / kGB;
-    const size_t npad = ((size_t)n + 3) & ~(size_t)3;
-    // rec (4n) | mm (npad) | part ((nb + kMaxG) x GP_N) | scalars (32)
-    return 4 * (size_t)n + npad + (size_t)(nb + kMaxG) * GP_N + 32;
-}
+extern "C" size_t svpf_grid_workspace_floats(int n) { return grid_ws_layout(n, nullptr, nullptr); }
 
 extern "C" size_t svpf_grid_lds_bytes(int n) {
-    int ps = 128;
-    while (ps < n) ps <<= 1;
-    return ((size_t)2 * ps + 4 + kMedianCands) * sizeof(float);
+    return std::max(merge_lds_bytes(n), std::max(round_lds_bytes(n), final_lds_bytes(n)));
 }
 
-// Raises grid_bandwidth's dynamic LDS limit once per process (not a stream
-// operation, so it is kept out of the launches a caller may capture).
+// Raises the dynamic LDS limits once per process (not a stream operation, so
+// it is kept out of the launches a caller may capture).
 extern "C" hipError_t svpf_grid_init(void) {
     static hipError_t done = hipErrorNotReady;
     if (done == hipSuccess) return done;
-    done = hipFuncSetAttribute((const void*)grid_bandwidth, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)svpf_grid_lds_bytes(kMaxN));
+    hipError_t e = hipFuncSetAttribute((const void*)grid_sort_merge, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)merge_lds_bytes(kMaxN));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)grid_median_round, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)round_lds_bytes(kMaxN));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)grid_median_final, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)final_lds_bytes(kMaxN));
+    done = e;
     return done;
 }
 
 #define SVPF_GRID_ITER(CSV)                                                                                       \
     case CSV:                                                                                                     \
-        grid_gradient<CSV><<<G, kGB, 0, stream>>>(P, S, gsc, h, mm, rec, lw, it, n, npad);                         \
-        grid_transport<CSV><<<G, kGB, 0, stream>>>(P, S, D, gsc, rec, h, v, part, it, t, n);                       \
+        grid_gradient<CSV><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, it, n, npad);                   \
+        grid_transport<CSV><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, it, t, n);                 \
         break;
 
-// Ticks [t0, t1) of one filter on `stream`: per tick 6 + 3 stein_max launches
+// Ticks [t0, t1) of one filter on `stream`: per tick 10 + med_rounds(n) + 3 stein_max launches
 // (stein_max bounds the device-side Stein budget).  h, hp, v, lw: the filter's
 // particle arrays [n]; ws: svpf_grid_workspace_floats(n) floats.
 extern "C" hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, float* h,
@@ -668,16 +1127,23 @@ extern "C" hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, co
     const int npad = (n + 3) & ~3;
     const int cs = grid_cs(n);
     const int G = (n + kGB / cs - 1) / (kGB / cs);
-    if (G > kMaxG) return hipErrorInvalidValue;
-    float4* rec = reinterpret_cast<float4*>(ws);
-    float* mm = ws + 4 * (size_t)n;
-    float* part = mm + npad;
-    GridScalars* gsc = reinterpret_cast<GridScalars*>(part + (size_t)(nb + kMaxG) * GP_N);
+    const int nmg = (n - 1 + kMedRows - 1) / kMedRows;
+    const int rounds = med_rounds(n);
+    if (G > kMaxG || nmg > kMaxMG) return hipErrorInvalidValue;
+    GridWs W;
+    grid_ws_layout(n, ws, &W);
     for (int t = t0; t < t1; ++t) {
-        grid_prologue<<<1, 64, 0, stream>>>(P, S, D, t, gsc);
-        grid_predict<<<nb, kGB, 0, stream>>>(P, S, D, t, gsc, h, hp, lw, part, n);
-        grid_bandwidth<<<1, 1024, svpf_grid_lds_bytes(n), stream>>>(P, S, gsc, h, part, nb, n);
-        grid_means<<<(npad + kGB - 1) / kGB, kGB, 0, stream>>>(P, gsc, hp, mm, n, npad);
+        grid_prologue<<<1, 64, 0, stream>>>(P, S, D, t, W.gsc);
+        grid_predict<<<nb, kGB, 0, stream>>>(P, S, D, t, W.gsc, h, hp, lw, W.part, n);
+        grid_bandwidth<<<1, 1024, 0, stream>>>(P, S, W.gsc, W.part, nb, n);
+        grid_sort_runs<<<(n + kRun - 1) / kRun, kRun, 0, stream>>>(P, h, W.runs, n);
+        grid_sort_merge<<<nb, kGB, merge_lds_bytes(n), stream>>>(P, W.runs, W.srt, n);
+        for (int r = 0; r < rounds; ++r)
+            grid_median_round<<<nmg, kMedRows, round_lds_bytes(n), stream>>>(P, S, W.srt, W.plo, W.phi, W.mstate,
+                                                                             W.mpart, W.mcand, W.gcount, r, n);
+        grid_median_final<<<1, 1024, final_lds_bytes(n), stream>>>(P, S, W.gsc, W.srt, W.plo, W.phi, W.mstate,
+                                                                   W.mpart, W.mcand, rounds, n);
+        grid_means<<<(npad + kGB - 1) / kGB, kGB, 0, stream>>>(P, W.gsc, hp, W.mm, n, npad);
         for (int it = 0; it < stein_max; ++it) {
             switch (cs) {
                 SVPF_GRID_ITER(1)
@@ -689,10 +1155,10 @@ extern "C" hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, co
                 SVPF_GRID_ITER(64)
                 default: return hipErrorInvalidValue;
             }
-            grid_ksd<<<1, kGB, 0, stream>>>(P, S, gsc, part, it, G, n);
+            grid_ksd<<<1, kGB, 0, stream>>>(P, S, W.gsc, W.part, it, G, n);
         }
-        grid_outputs<<<nb, kGB, 0, stream>>>(P, S, D, t, gsc, h, lw, part, n);
-        grid_finish<<<1, kGB, 0, stream>>>(P, S, D, t, gsc, part, nb, n);
+        grid_outputs<<<nb, kGB, 0, stream>>>(P, S, D, t, W.gsc, h, lw, W.part, n);
+        grid_finish<<<1, kGB, 0, stream>>>(P, S, D, t, W.gsc, W.part, nb, n);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

@@ -37,6 +37,27 @@ def test_grid_one_tick_from_identical_state(torch_cuda, n):
     compare(rec, out)
 
 
+def test_grid_median_heavy_ties(torch_cuda):
+    """Predicted particles with heavy ties (initial values rounded to 0.1, zero
+    predict and Langevin noise): the selection needs more rounds than the grid
+    launches at this size, so the final workgroup finishes it; the bandwidth
+    stays bit-exact and the tick matches the oracle."""
+    n = 3000
+    cfg = config.plain_config(n=n)
+    rng = np.random.default_rng(41)
+    h0 = np.round(rng.standard_normal(n) * 0.6 - 1.0, 1).astype(np.float32)
+    noise = {"z_pred": np.zeros((1, n), np.float32), "xi": np.zeros((1, 5, n), np.float32)}
+    y = np.array([0.3], dtype=np.float32)
+    print("grid median heavy ties", flush=True)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0, path="grid")
+    hp = d2h(f.state.h_pred, n)
+    assert np.unique(hp).size < n // 10
+    assert np.float32(rec[0, 7]) == oracle.median_r2_f32(hp)
+    o, out = run_oracle(cfg, y, noise, h0)
+    assert np.max(np.abs(f.particles() - o.h)) <= 1e-4
+    compare(rec, out)
+
+
 @pytest.mark.parametrize("name", sorted(FEATURE_CONFIGS))
 def test_grid_feature_configs_short_run(torch_cuda, name):
     n, T = 300, 25
