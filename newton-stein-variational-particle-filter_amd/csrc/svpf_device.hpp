@@ -30,6 +30,9 @@ namespace svpf {
 
 constexpr int kWave = 64;
 constexpr int kMedianCands = 32;  // candidate pairs gathered for the final rank selection
+// thresholds counted per selection round (15 measured slower inside one workgroup:
+// the wider lockstep searches cost more per round than the rounds they save)
+constexpr int kMedianW = 7;
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy
@@ -470,10 +473,10 @@ SVPF_DEV void row_search(const float* s, int n, float si, int a, int b, const ui
 // rank is read off by counting.  Scratch (LDS): plo, phi (n ints each), cand
 // (cmax floats), red (48 + BS/64 + 1 words); Idx may be a 16-bit type when n < 65536.  The first KR row slots of each thread keep
 // their search results in registers; later slots search again.
-template <int BS, int KR, typename Idx = int>
+template <int BS, int KR, typename Idx = int, int W = kMedianW>
 SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi, float* cand, int cmax,
                                   uint32_t* red) {
-    constexpr int W = 7;
+    static_assert(W <= 16, "counts: three rotating 16-word buffers");
     constexpr int NW = BS / kWave;
     constexpr uint32_t kSeedStep = 1u << 19;  // 1/16 octave of r^2 (in float bit patterns)
     enum { SEED = 0, DOWN = 1, UP = 2, SPLIT = 3 };
