@@ -96,7 +96,8 @@ struct DiffLds {
 template <int BS, int R>
 struct DiffEngine {
     float4* rec;    // [nmax] {h/bw, s, c/bw, w}; sort input and median scratch in the forward bandwidth
-    float4* recb;   // [nmax] pass-B records {scaled h, log2 normaliser, a, E dd}; predicted h in the pair search
+    float4* recb;   // [nmax] pass-B records {scaled h, log2 normaliser, a, kappa E dd}; predicted h in the pair
+                    // search; the sorted means during the forward Stein loop
     float* mbuf;    // [p2] scaled mixture means (padded with far sentinels) / sorted positions
     float* red;     // [64] reduction scratch
     float* scal;    // [16] broadcast scalars
@@ -144,39 +145,51 @@ struct DiffEngine {
         return mx + logf(fmaxf(se[0] * inv_n, 1e-10f));
     }
 
+    // ---- sorted copy of the means (recb is free during the forward Stein loop) ----
+    // The means do not change during a tick's Stein loop.  With a sorted copy
+    // the shift min_i fl(hh - mm_i)^2 (the reference's max term) is a binary
+    // search: rounding is monotone, so it is attained at one of the two means
+    // around hh and is bit-identical to the minimum over all of them.  The
+    // sums still run over mbuf in its own order.
+    SVPF_DEV void sort_means_copy() {
+        float* src = reinterpret_cast<float*>(rec);
+        __syncthreads();  // mbuf complete
+        for (int i = (int)threadIdx.x; i < n; i += BS) src[i] = mbuf[i];
+        __syncthreads();
+        block_sort<BS>(src, reinterpret_cast<float*>(recb), n);
+    }
+    SVPF_DEV float nearest_sq(float hh) const {
+        const float* ms = reinterpret_cast<const float*>(recb);
+        int pos = -1;  // last index with ms[pos] < hh
+        for (int step = 1 << (31 - __clz(n)); step > 0; step >>= 1) {
+            const int q = pos + step;
+            pos = (q < n && ms[q] < hh) ? q : pos;
+        }
+        float best = 3.0e38f;
+        if (pos >= 0) {
+            const float d = hh - ms[pos];
+            best = d * d;
+        }
+        if (pos + 1 < n) {
+            const float d = hh - ms[pos + 1];
+            best = fminf(best, d * d);
+        }
+        return best;
+    }
+
     // ---- mixture-prior sums of own rows over all components (scaled units) ----
     template <bool MOM>
     SVPF_DEV void prior_sums(const float (&hh)[R], float (&off)[R], float (&sr)[R], float (&s1)[R], float (&s2)[R],
                              float (&s3)[R]) const {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            off[r] = 3.0e38f;
+            off[r] = nearest_sq(hh[r]);
             sr[r] = s1[r] = s2[r] = s3[r] = 0.f;
         }
         const int nq = (n + 3) >> 2;
         const float4* m4 = reinterpret_cast<const float4*>(mbuf);
         constexpr int U = 2;
         int q = 0;
-        for (; q + U <= nq; q += U) {
-            float4 m[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) m[u] = m4[q + u];
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    float d0 = hh[r] - m[u].x, d1 = hh[r] - m[u].y, d2 = hh[r] - m[u].z, d3 = hh[r] - m[u].w;
-                    off[r] = fminf(off[r], fminf(fminf(d0 * d0, d1 * d1), fminf(d2 * d2, d3 * d3)));
-                }
-        }
-        for (; q < nq; ++q) {
-            const float4 m = m4[q];
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                float d0 = hh[r] - m.x, d1 = hh[r] - m.y, d2 = hh[r] - m.z, d3 = hh[r] - m.w;
-                off[r] = fminf(off[r], fminf(fminf(d0 * d0, d1 * d1), fminf(d2 * d2, d3 * d3)));
-            }
-        }
         auto acc = [&](const float4& m) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -341,6 +354,7 @@ struct DiffEngine {
         for (int r = 0; r < R; ++r)
             if (valid(r)) mbuf[row(r)] = sq * m[r];
         for (int i = n + (int)threadIdx.x; i < p2; i += BS) mbuf[i] = 1.0e18f;  // exp2 underflows to 0
+        sort_means_copy();
 
         const float sgn2 = C.paper_sign ? 2.0f : -2.0f;
         const float temp = C.temperature;
@@ -419,7 +433,7 @@ struct DiffEngine {
                 const float K = __builtin_amdgcn_rcpf(fmaf(d, d, 1.0f));
                 const float K2 = K * K;
                 const float dK2 = d * K2;
-                const float tt = fmaf(-4.0f * d * K, dK2, K2);  // K^2 (1 - 4 d^2 K)
+                const float tt = K2 * fmaf(4.0f, K, -3.0f);  // K^2 (1 - 4 d^2 K), d^2 K = 1 - K
                 R1[r] = fmaf(e.y, dK2, R1[r]);
                 R2[r] = fmaf(e.z, tt, R2[r]);
                 C1[r] = fmaf(e.w, K, C1[r]);
@@ -450,7 +464,8 @@ struct DiffEngine {
                 const float dd = e.x - mm[r];
                 const float rt = __builtin_amdgcn_exp2f(fmaf(-dd, dd, -e.y));
                 const float ar = e.z * rt;
-                acc[r] = fmaf(ar, fmaf(-kDiffKappa * dd, dd - e.w, 1.0f), acc[r]);
+                // 1 - kappa dd (dd - Edd) = 1 + dd (kappa Edd - kappa dd); e.w = kappa Edd
+                acc[r] = fmaf(ar, fmaf(fmaf(-kDiffKappa, dd, e.w), dd, 1.0f), acc[r]);
             }
         };
         int j = 0;
@@ -582,7 +597,7 @@ struct DiffEngine {
             }
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                if (valid(r)) recb[row(r)] = make_float4(h[r] * sq, L[r], a[r], Edd[r]);
+                if (valid(r)) recb[row(r)] = make_float4(h[r] * sq, L[r], a[r], kDiffKappa * Edd[r]);
             __syncthreads();
             pass_b(mm, mb);
         }
