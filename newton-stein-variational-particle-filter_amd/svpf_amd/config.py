@@ -29,7 +29,7 @@ INT_FIELDS = (
     "use_linear_beta", "stein_min_steps", "stein_max_steps", "use_mim",
     "use_asymmetric_rho", "use_local_params", "use_guided", "use_guide",
     "use_guide_preserving", "use_adaptive_guide", "use_rejuvenation", "use_fan_mode",
-    "use_adaptive_mu", "use_adaptive_sigma", "use_ksd_early_stop",
+    "use_adaptive_mu", "use_adaptive_sigma", "use_ksd_early_stop", "use_heun", "use_student_t_state",
 )
 FLOAT_FIELDS = (
     "mu", "rho", "sigma_z", "gamma", "nu", "lik_offset", "temperature", "rmsprop_rho",
@@ -38,7 +38,7 @@ FLOAT_FIELDS = (
     "guide_strength_base", "guide_strength_max", "guide_innovation_threshold",
     "rejuv_ksd_threshold", "rejuv_prob", "rejuv_blend", "mu_process_var",
     "mu_obs_var_scale", "mu_min", "mu_max", "sigma_boost_threshold", "sigma_boost_max",
-    "ksd_improvement_threshold",
+    "ksd_improvement_threshold", "nu_state",
 )
 
 # src/svpf_optimized_graph.cu:110-302 (svpf_create) with SVPFParams defaults of
@@ -58,6 +58,7 @@ REFERENCE_DEFAULTS = dict(
     rejuv_ksd_threshold=0.30, rejuv_prob=0.30, rejuv_blend=0.30, mu_process_var=0.001,
     mu_obs_var_scale=10.0, mu_min=-6.0, mu_max=-1.0, sigma_boost_threshold=1.0,
     sigma_boost_max=3.0, ksd_improvement_threshold=0.05,
+    use_heun=0, use_student_t_state=0, nu_state=5.0,
 )
 
 

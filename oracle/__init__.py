@@ -21,7 +21,7 @@ CFG_INT_FIELDS = (
     "use_linear_beta", "stein_min_steps", "stein_max_steps", "use_mim",
     "use_asymmetric_rho", "use_local_params", "use_guided", "use_guide",
     "use_guide_preserving", "use_adaptive_guide", "use_rejuvenation", "use_fan_mode",
-    "use_adaptive_mu", "use_adaptive_sigma", "use_ksd_early_stop", "pad_",
+    "use_adaptive_mu", "use_adaptive_sigma", "use_ksd_early_stop", "use_heun", "use_student_t_state", "pad_",
 )
 CFG_DOUBLE_FIELDS = (
     "mu", "rho", "sigma_z", "gamma", "nu", "lik_offset", "temperature", "rmsprop_rho",
@@ -30,7 +30,7 @@ CFG_DOUBLE_FIELDS = (
     "guide_strength_base", "guide_strength_max", "guide_innovation_threshold",
     "rejuv_ksd_threshold", "rejuv_prob", "rejuv_blend", "mu_process_var",
     "mu_obs_var_scale", "mu_min", "mu_max", "sigma_boost_threshold", "sigma_boost_max",
-    "ksd_improvement_threshold",
+    "ksd_improvement_threshold", "nu_state",
 )
 
 
@@ -111,6 +111,8 @@ def lib():
         L.svpfo_u01_from_bits.argtypes = [ctypes.c_uint32]
         L.svpfo_normal_from_bits.restype = D
         L.svpfo_normal_from_bits.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.svpfo_student_t_draw.restype = D
+        L.svpfo_student_t_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, D]
         _lib = L
     return _lib
 

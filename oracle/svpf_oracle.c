@@ -68,7 +68,7 @@ double svpfo_normal_from_bits(uint32_t a, uint32_t b) {
 }
 
 /* Noise purposes (third counter word). */
-enum { PURPOSE_PREDICT = 0, PURPOSE_REJUV = 1, PURPOSE_INIT = 255, PURPOSE_SVLD0 = 16 };
+enum { PURPOSE_PREDICT = 0, PURPOSE_REJUV = 1, PURPOSE_INIT = 255, PURPOSE_SVLD0 = 16, PURPOSE_STATE_T0 = 128 };
 
 static void philox_draw(uint64_t seed, uint32_t filter, uint32_t tick, uint32_t purpose, uint32_t i,
                         uint32_t out[4]) {
@@ -82,6 +82,26 @@ static void philox_draw(uint64_t seed, uint32_t filter, uint32_t tick, uint32_t 
 /* ------------------------------------------------------------------------ */
 
 /* src/svpf_optimized_graph.cu:123-125 */
+/* src/svpf_opt_kernels.cu:31-53: t = z sqrt(nu_int / chi2), chi2 the sum of
+ * nu_int squared normals, nu_int = max(round(nu), 3); a normal for nu > 30.
+ * z is the predict normal (purpose 0, as the Gaussian noise); normal k of the
+ * chi-squared sum is the cosine normal of purpose 128 + k. */
+double svpfo_student_t_draw(uint64_t seed, uint32_t filter, uint32_t tick, uint32_t i, double nu_state) {
+    uint32_t o[4];
+    philox_draw(seed, filter, tick, PURPOSE_PREDICT, i, o);
+    double z = svpfo_normal_from_bits(o[0], o[1]);
+    if (nu_state > 30.0) return z;
+    int nu_int = (int)(nu_state + 0.5);
+    if (nu_int < 3) nu_int = 3;
+    double chi2 = 0.0;
+    for (int k = 0; k < nu_int; ++k) {
+        philox_draw(seed, filter, tick, PURPOSE_STATE_T0 + (uint32_t)k, i, o);
+        double u = svpfo_normal_from_bits(o[0], o[1]);
+        chi2 += u * u;
+    }
+    return z * sqrt((double)nu_int / (chi2 + 1e-8));
+}
+
 double svpfo_student_t_const(double nu) {
     return lgamma((nu + 1.0) / 2.0) - lgamma(nu / 2.0) - 0.5 * log(M_PI * nu);
 }
@@ -224,17 +244,41 @@ void svpfo_gradient(const svpfo_cfg* c, const double* h, const double* h_prev, d
     double inv_2s2 = 0.5 / s2, inv_s2 = 1.0 / s2;
     for (int j = 0; j < n; ++j) {
         double hj = h[j];
-        double lmax = -1e10;
-        for (int i = 0; i < n; ++i) { double d = hj - m[i]; double l = -d * d * inv_2s2; if (l > lmax) lmax = l; }
-        double sum_r = 0.0, wg = 0.0;
-        for (int i = 0; i < n; ++i) {
-            double d = hj - m[i];
-            double r = exp(-d * d * inv_2s2 - lmax);
-            sum_r += r;
-            wg -= r * d * inv_s2;
+        double g_prior, h_prior;
+        if (c->use_student_t_state) {
+            /* :406-440: components log r_i = -(nu_s+1)/2 log(1 + d^2/(nu_s sigma^2)),
+             * gradient -(nu_s+1) d/(nu_s sigma^2 + d^2), Hessian the r-weighted mean of
+             * -(nu_s+1)(nu_s sigma^2 - d^2)/(nu_s sigma^2 + d^2)^2 */
+            double ns2 = c->nu_state * s2, np1 = c->nu_state + 1.0;
+            double lmax = -1e10;
+            for (int i = 0; i < n; ++i) {
+                double d = hj - m[i];
+                double l = -0.5 * np1 * log(1.0 + d * d / ns2);
+                if (l > lmax) lmax = l;
+            }
+            double sum_r = 0.0, wg = 0.0, wh = 0.0;
+            for (int i = 0; i < n; ++i) {
+                double d = hj - m[i], den = ns2 + d * d;
+                double r = exp(-0.5 * np1 * log(1.0 + d * d / ns2) - lmax);
+                sum_r += r;
+                wg -= r * np1 * d / den;
+                wh += r * (-np1 * (ns2 - d * d) / (den * den));
+            }
+            g_prior = wg / (sum_r + 1e-8);
+            h_prior = wh / (sum_r + 1e-8);
+        } else {
+            double lmax = -1e10;
+            for (int i = 0; i < n; ++i) { double d = hj - m[i]; double l = -d * d * inv_2s2; if (l > lmax) lmax = l; }
+            double sum_r = 0.0, wg = 0.0;
+            for (int i = 0; i < n; ++i) {
+                double d = hj - m[i];
+                double r = exp(-d * d * inv_2s2 - lmax);
+                sum_r += r;
+                wg -= r * d * inv_s2;
+            }
+            g_prior = wg / (sum_r + 1e-8);
+            h_prior = -inv_s2;
         }
-        double g_prior = wg / (sum_r + 1e-8);
-        double h_prior = -inv_s2;
 
         double vol = safe_exp(hj);
         double ysq = y_t * y_t;
@@ -421,7 +465,9 @@ int svpfo_tick(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, d
     /* 2. predict (src/svpf_opt_kernels.cu:125-276) */
     for (int i = 0; i < n; ++i) {
         double hi = h[i], hpi = h_prev[i];
-        double z = noise_normal(nz, nz->z_pred, i, tick, PURPOSE_PREDICT);
+        double z = (rng && c->use_student_t_state)
+                       ? svpfo_student_t_draw(nz->seed, nz->filter_id, tick, (uint32_t)i, c->nu_state)
+                       : noise_normal(nz, nz->z_pred, i, tick, PURPOSE_PREDICT);
         double lev = c->gamma * y_prev / (safe_exp(hi / 2.0) + 1e-8);
         double hnew;
         if (c->use_guided || c->use_mim) {
@@ -513,11 +559,50 @@ int svpfo_tick(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, d
             int last = (ai == n_anneal - 1) && (s == si - 1);
             int row = total;
             total++;
+            int want_ksd = last || c->use_ksd_early_stop;
+            double eff = base_step * bf;
+            if (c->use_heun) {
+                /* src/svpf_optimized_graph.cu:1147-1232, src/svpf_opt_kernels.cu:1293-1566.  The
+                 * Stein operator takes the raw score (plain form, or the kernel-weighted Hessian of
+                 * full Newton); predictor: Euler step from h with the current RMSProp state and no
+                 * noise; corrector: from h with the mean of the two operators, the RMSProp update
+                 * and the Langevin noise; KSD at the original positions with the score of the
+                 * predicted ones. */
+                svpfo_cfg ch = *c;
+                ch.use_newton = full;
+                ch.use_full_newton = full;
+                svpfo_cfg cp = *c;
+                cp.use_newton = 0;
+                cp.use_full_newton = 0;
+                double* ho = (double*)malloc((size_t)n * sizeof(double));
+                double* phi2 = (double*)malloc((size_t)n * sizeof(double));
+                memcpy(ho, h, (size_t)n * sizeof(double));
+                svpfo_gradient(c, h, h_prev, y_t, y_prev, mu, sigma, beta, grad, log_w, prec, curv);
+                svpfo_stein_phi(&ch, h, grad, curv, inv_bw_sq, phi, NULL);
+                for (int i = 0; i < n; ++i) h[i] = clamp_logvol(ho[i] + eff * phi[i] / sqrt(v[i] + c->rmsprop_eps));
+                svpfo_gradient(c, h, h_prev, y_t, y_prev, mu, sigma, beta, grad, log_w, prec, curv);
+                svpfo_stein_phi(&ch, h, grad, curv, inv_bw_sq, phi2, NULL);
+                if (want_ksd) svpfo_stein_phi(&cp, ho, grad, curv, inv_bw_sq, hn, kp);
+                for (int i = 0; i < n; ++i) {
+                    double p = 0.5 * (phi[i] + phi2[i]);
+                    double vn = c->rmsprop_rho * v[i] + (1.0 - c->rmsprop_rho) * p * p;
+                    v[i] = vn;
+                    double drift = eff * p / sqrt(vn + c->rmsprop_eps);
+                    double diff = 0.0;
+                    if (temp > 1e-6) {
+                        double xi;
+                        if (rng) xi = noise_normal(nz, NULL, i, tick, PURPOSE_SVLD0 + (uint32_t)row);
+                        else xi = (row < nz->xi_rows) ? (double)nz->xi[(size_t)row * n + i] : 0.0;
+                        diff = sqrt(2.0 * eff * temp) * xi;
+                    }
+                    hn[i] = clamp_logvol(ho[i] + drift + diff);
+                }
+                free(ho);
+                free(phi2);
+            } else {
             svpfo_gradient(c, h, h_prev, y_t, y_prev, mu, sigma, beta, grad, log_w, prec, curv);
             const double* sc = (use_newton && !full) ? prec : grad;
-            int want_ksd = last || c->use_ksd_early_stop;
             svpfo_stein_phi(c, h, sc, curv, inv_bw_sq, phi, want_ksd ? kp : NULL);
-            double eff = base_step * bf;
             for (int i = 0; i < n; ++i) {
                 double p = phi[i];
                 double vn = c->rmsprop_rho * v[i] + (1.0 - c->rmsprop_rho) * p * p;
@@ -531,6 +616,7 @@ int svpfo_tick(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, d
                     diff = sqrt(2.0 * eff * temp) * xi;
                 }
                 hn[i] = clamp_logvol(h[i] + drift + diff);
+            }
             }
             memcpy(h, hn, (size_t)n * sizeof(double));
             if (want_ksd) {

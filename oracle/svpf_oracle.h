@@ -52,6 +52,8 @@ typedef struct svpfo_cfg {
     int use_adaptive_mu;
     int use_adaptive_sigma;
     int use_ksd_early_stop;
+    int use_heun;              /* Heun predictor-corrector Stein step */
+    int use_student_t_state;   /* Student-t state noise and transition prior */
     int pad_;
     double mu, rho, sigma_z, gamma;
     double nu, lik_offset;
@@ -64,6 +66,7 @@ typedef struct svpfo_cfg {
     double mu_process_var, mu_obs_var_scale, mu_min, mu_max;
     double sigma_boost_threshold, sigma_boost_max;
     double ksd_improvement_threshold;
+    double nu_state;           /* Student-t state degrees of freedom */
 } svpfo_cfg;
 
 /* Scalar filter state carried between ticks (host fields of SVPFState and the
@@ -147,6 +150,8 @@ int svpfo_run(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, do
 
 /* Counter-based generator: Philox4x32-10 (Salmon et al., SC'11). */
 void svpfo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Student-t state noise from the counter generator (particle i, tick, filter). */
+double svpfo_student_t_draw(uint64_t seed, uint32_t filter, uint32_t tick, uint32_t i, double nu_state);
 double svpfo_u01_from_bits(uint32_t x);
 double svpfo_normal_from_bits(uint32_t a, uint32_t b);
 
