@@ -166,7 +166,7 @@ typedef struct {
     float ksd_prev;
     int stein_steps_used;
 
-    int use_student_t_state; /* rejected: SVPF_ERR_UNSUPPORTED */
+    int use_student_t_state; /* single-workgroup kernel only (n <= its capacity); else SVPF_ERR_UNSUPPORTED */
     float nu_state;
 
     int use_smoothing;
@@ -179,7 +179,7 @@ typedef struct {
 
     int use_persistent_kernel; /* 1: cooperative multi-workgroup kernel (default for n >= 256) */
     int persistent_kernel_supported;
-    int use_heun;      /* rejected: SVPF_ERR_UNSUPPORTED */
+    int use_heun;      /* single-workgroup kernel only (n <= its capacity); else SVPF_ERR_UNSUPPORTED */
     int use_two_factor; /* rejected: SVPF_ERR_UNSUPPORTED */
 
     /* extensions (defaults reproduce the reference) */
@@ -314,7 +314,9 @@ static inline int svpf_get_fan_mode(const SVPFState* state) { return state->use_
  * filter indices grouped by tile; group g covers order[group_start[g] ..
  * group_start[g+1]) and runs with tile group_tile[g] and block size
  * group_bs[g] on grid group_grid[g] workgroups.  queues holds one int per
- * group. */
+ * group.  A group holding a filter with the Heun step or the Student-t state
+ * (SVPF_F_HEUN / SVPF_F_STUDENT_STATE) ORs SVPF_TILE_EXTENDED into its tile id;
+ * without it such a filter is rejected (its svpf_fstate status is set). */
 int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, float* d_pool_h, float* d_pool_hp,
                      float* d_pool_v, const int* d_order, const int* group_start, const int* group_tile,
                      const int* group_bs, const int* group_grid, const int* group_nmax, int n_groups, int* d_queues,
@@ -349,7 +351,9 @@ int svpf_diff2d_forward(const void* config, const void* d_desc, int B, hipStream
 int svpf_diff2d_backward(const void* config, const void* d_desc, int B, hipStream_t stream);
 
 /* Largest particle count a tile of the given block size holds (0: unknown tile).
- * Block sizes 256 and 512 take tiles 0-7, 1024 takes tiles 0-4. */
+ * Block sizes 256 and 512 take tiles 0-7, 1024 takes tiles 0-4, each with or
+ * without SVPF_TILE_EXTENDED. */
+#define SVPF_TILE_EXTENDED 0x100
 int svpf_tile_capacity_query(int tile, int block_size);
 /* Workgroups of that tile resident per CU for filters of up to nmax particles
  * (HIP occupancy of the kernel with its LDS; 0: unknown tile).  Sizes the

@@ -35,6 +35,7 @@ constexpr int kBlockBytes = 1024;
 constexpr int kUploadBytes = kBlockDesc + (int)sizeof(svpf_fdesc);
 static_assert(sizeof(svpf_fparams) == 256, "fparams layout");
 static_assert(sizeof(svpf_fstate) == 128, "fstate layout");
+static_assert(SVPF_TILE_EXT == SVPF_TILE_EXTENDED, "extended tile bit");
 static_assert(kBlockDesc + sizeof(svpf_fdesc) <= (size_t)kBlockBytes, "block layout");
 
 }  // namespace
@@ -205,6 +206,9 @@ void pack_params(const SVPFState* s, const SVPFParams* p, svpf_fparams* o) {
     if (s->use_adaptive_sigma) f |= SVPF_F_ADAPT_SIGMA;
     if (s->use_ksd_early_stop) f |= SVPF_F_KSD_EARLY_STOP;
     if (s->stein_repulsive_sign == SVPF_STEIN_SIGN_PAPER) f |= SVPF_F_PAPER_SIGN;
+    if (s->use_heun) f |= SVPF_F_HEUN;
+    if (s->use_student_t_state) f |= SVPF_F_STUDENT_STATE;
+    o->nu_state = s->nu_state;
     o->n = s->n_particles;
     o->flags = f;
     o->n_anneal = s->n_anneal_steps;
@@ -315,7 +319,10 @@ void fill_desc(SVPFState* s, svpf_fdesc* d, const float* y, float* out, int tick
 }
 
 int check_supported(SVPFState* s) {
-    if (s->use_student_t_state || s->use_heun || s->use_two_factor) return SVPF_ERR_UNSUPPORTED;
+    if (s->use_two_factor) return SVPF_ERR_UNSUPPORTED;
+    // Heun and the Student-t state run in the single-workgroup engine only
+    if ((s->use_heun || s->use_student_t_state) && (s->impl->tile < 0 || s->impl->grid_mode == SVPF_GRID_ON))
+        return SVPF_ERR_UNSUPPORTED;
     if (s->n_particles != s->impl->n) return SVPF_ERR_INVALID_ARGUMENT;
     if (s->bandwidth_mode < 0 || s->bandwidth_mode > 2) return SVPF_ERR_INVALID_ARGUMENT;
     if (s->n_anneal_steps < 1 && s->use_annealing) return SVPF_ERR_INVALID_ARGUMENT;
@@ -328,7 +335,8 @@ int check_supported(SVPFState* s) {
 // for it (use_persistent_kernel) and the features it implements suffice.
 bool coop_applies(const SVPFState* s) {
     const svpf_impl* I = s->impl;
-    return s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop;
+    return s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop && !s->use_heun &&
+           !s->use_student_t_state;
 }
 
 // Path of one filter's step.  Forced by svpf_set_grid_path, or by a filter above
@@ -342,6 +350,7 @@ constexpr int kCoopMinN = 256;
 bool use_grid(const SVPFState* s) {
     const svpf_impl* I = s->impl;
     if (!I->dgrid_ws) return false;
+    if (s->use_heun || s->use_student_t_state) return I->tile < 0;  // single-workgroup engine features
     if (I->tile < 0 || I->grid_mode == SVPF_GRID_ON) return true;
     if (I->grid_mode == SVPF_GRID_OFF) return false;
     if (I->n >= kGridAutoMinN) return true;
@@ -362,7 +371,8 @@ hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
     if (use_coop(s))
         return svpf_launch_coop(I->coop_cs, I->coop_G, I->coop_stride, I->dp(), I->ds(), I->dd(), I->d_h(),
                                 I->d_hp(), I->d_v(), I->xbuf, I->n, t0, t1, 1, cs);
-    return svpf_launch_batched(I->bs, I->tile, 1, I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->dorder,
+    const int tile = I->tile | ((s->use_heun || s->use_student_t_state) ? SVPF_TILE_EXT : 0);
+    return svpf_launch_batched(I->bs, tile, 1, I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->dorder,
                                1, nullptr, I->n, t0, t1, cs);
 }
 
@@ -712,7 +722,11 @@ void svpf_initialize(SVPFState* s, const SVPFParams* p, unsigned long long seed)
     s->timestep = 0;
     s->y_prev = 0.0f;
     I->seed = seed;
+    // nu_state >= 2.5 keeps the Student-t state variance finite; its stationary
+    // variance is nu / (nu - 2) times the Gaussian one (src/svpf_optimized_graph.cu:366-383)
+    s->nu_state = fmaxf(s->nu_state, 2.5f);
     float base_var = p->sigma_z * p->sigma_z / (1.0f - p->rho * p->rho + 1e-6f);
+    if (s->use_student_t_state) base_var *= s->nu_state / (s->nu_state - 2.0f);
     float sd = sqrtf(base_var);
     hipError_t e = svpf_launch_init(I->d_h(), I->d_hp(), I->d_v(), I->n, p->mu, sd, seed, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);

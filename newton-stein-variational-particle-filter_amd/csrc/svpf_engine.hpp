@@ -68,7 +68,9 @@ struct FilterIO {
     float* v;
 };
 
-template <int BS, int R, int CS>
+// EXT: instantiations with the Heun step and the Student-t state (SVPF_F_HEUN,
+// SVPF_F_STUDENT_STATE); without it those flags are not read.
+template <int BS, int R, int CS, bool EXT = false>
 struct Engine {
     static constexpr int ROWS_PER_PASS = BS / CS;
     static constexpr int CAP = R * ROWS_PER_PASS;
@@ -244,6 +246,52 @@ struct Engine {
         }
     }
 
+    // Student-t state prior (src/svpf_opt_kernels.cu:406-440): components
+    // log r_i = -a log(1 + d^2/(nu_s sigma^2)), a = (nu_s + 1)/2, shifted by the
+    // nearest one.  In units scaled by st = 1/sqrt(nu_s sigma^2) (mbuf holds the
+    // means scaled by st, sorted) with q_i = 1 + (hh - mm_i)^2:
+    //   r_i = exp2(a (log2 q_min - log2 q_i)),
+    //   sr = sum r_i, sg = sum r_i dd_i / q_i, sh = sum r_i (1 - dd_i^2) / q_i^2,
+    // so g_prior = -(nu_s+1) st sg/sr and the prior Hessian -(nu_s+1) st^2 sh/sr.
+    // A pair costs log2 + exp2 + rcp and five FMA-class operations.
+    SVPF_DEV void prior_sums_t(float st, float a, float (&sr)[R], float (&sg)[R], float (&sh)[R]) const {
+        float hh[R], base[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            hh[r] = h[r] * st;
+            base[r] = a * __builtin_amdgcn_logf(1.0f + nearest_sq(hh[r]));
+            sr[r] = sg[r] = sh[r] = 0.f;
+        }
+        const int nq = (n + 3) >> 2;
+        const float4* m4 = reinterpret_cast<const float4*>(mbuf);
+        auto one = [&](int r, float m) {
+            const float d = hh[r] - m;
+            const float q = fmaf(d, d, 1.0f);
+            const float iq = __builtin_amdgcn_rcpf(q);
+            const float e = __builtin_amdgcn_exp2f(fmaf(-a, __builtin_amdgcn_logf(q), base[r]));
+            const float u = e * iq;
+            sr[r] += e;
+            sg[r] = fmaf(u, d, sg[r]);
+            sh[r] = fmaf(u * iq, 2.0f - q, sh[r]);
+        };
+        for (int q = cs; q < nq; q += CS) {
+            const float4 m = m4[q];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                one(r, m.x);
+                one(r, m.y);
+                one(r, m.z);
+                one(r, m.w);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            sr[r] = group_sum<CS>(sr[r]);
+            sg[r] = group_sum<CS>(sg[r]);
+            sh[r] = group_sum<CS>(sh[r]);
+        }
+    }
+
     // ---------------------------------------------------------------------
     // IMQ Stein transport (src/svpf_opt_kernels.cu:527-599 plain,
     // :742-906 local Newton, :912-1103 full Newton) with the reference KSD
@@ -408,6 +456,17 @@ struct Engine {
                     rng.draw((uint32_t)i, tick_id, SVPF_PURPOSE_PREDICT, o);
                     z = normal_from_bits(o[0], o[1]);
                     u = u01_from_bits(o[2]);
+                    if (EXT && (F & SVPF_F_STUDENT_STATE) && P.nu_state <= 30.0f) {
+                        // Student-t state noise (src/svpf_opt_kernels.cu:31-53): z sqrt(k / chi2_k),
+                        // k = max(round(nu_s), 3) squared normals of their own purposes
+                        const int k = max((int)(P.nu_state + 0.5f), 3);
+                        float chi2 = 0.0f;
+                        for (int c = 0; c < k; ++c) {
+                            const float w = rng.normal((uint32_t)i, tick_id, SVPF_PURPOSE_STATE_T0 + (uint32_t)c);
+                            chi2 = fmaf(w, w, chi2);
+                        }
+                        z *= sqrtf((float)k / (chi2 + 1e-8f));
+                    }
                 } else {
                     z = D.z_pred[(size_t)t * n + i];
                     if (D.u_pred) u = D.u_pred[(size_t)t * n + i];
@@ -475,7 +534,8 @@ struct Engine {
         // ---- mixture-component means m_i from the previous tick's particles,
         //      scaled by sq ----
         const float inv_s2 = 1.0f / (sigma * sigma);
-        const float sq = sqrtf(1.4426950408889634f * 0.5f * inv_s2);
+        const float sq = (EXT && (F & SVPF_F_STUDENT_STATE)) ? sqrtf(inv_s2 / P.nu_state)
+                                                    : sqrtf(1.4426950408889634f * 0.5f * inv_s2);
         {
             const float lev_num = P.gamma * y_prev;
             sort_means([&](int r) { return sq * (mu + P.rho * (hp[r] - mu) + lev_num / (expf(hp[r] * 0.5f) + 1e-8f)); });
@@ -507,19 +567,47 @@ struct Engine {
                 const bool want_ksd = last || early;
                 const int it_row = total;
                 ++total;
-                __syncthreads();  // previous transport finished reading hsc; mbuf complete
-                gradient(P, beta, y_t, inv_s2, sq, rs, sgn2, mode);
-                __syncthreads();  // hsc complete
-                float phi[R], ksdp[R];
-                if (mode == MODE_NEWTON) {
-                    if (want_ksd) phi_rows<MODE_NEWTON, true>(ib, rs, inv_n, phi, ksdp);
-                    else phi_rows<MODE_NEWTON, false>(ib, rs, inv_n, phi, ksdp);
-                } else if (mode == MODE_FULL) {
-                    if (want_ksd) phi_rows<MODE_FULL, true>(ib, rs, inv_n, phi, ksdp);
-                    else phi_rows<MODE_FULL, false>(ib, rs, inv_n, phi, ksdp);
+                float phi[R], ksdp[R], ho[R];
+                if (EXT && (F & SVPF_F_HEUN)) {
+                    // Heun predictor-corrector (src/svpf_optimized_graph.cu:1147-1232,
+                    // src/svpf_opt_kernels.cu:1293-1566): the operator takes the raw score
+                    // (plain, or full Newton's kernel-weighted curvature); predictor from h
+                    // with the current RMSProp state and no noise; the corrector below
+                    // moves h with the mean of both operators.  KSD: original positions,
+                    // scores of the predicted ones.
+                    const int hmode = (mode == MODE_FULL) ? MODE_FULL : MODE_PLAIN;
+                    float phi1[R], unused[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) ho[r] = h[r];
+                    __syncthreads();  // previous transport finished reading hsc
+                    gradient(P, beta, y_t, inv_s2, sq, rs, sgn2, hmode);
+                    __syncthreads();
+                    phi_dispatch(hmode, false, ib, rs, inv_n, phi1, unused);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) h[r] = clamp_logvol(ho[r] + eff * phi1[r] * rsqrtf(v[r] + P.rms_eps));
+                    __syncthreads();
+                    gradient(P, beta, y_t, inv_s2, sq, rs, sgn2, hmode);
+                    __syncthreads();
+                    phi_dispatch(hmode, false, ib, rs, inv_n, phi, unused);
+                    if (want_ksd) {
+                        __syncthreads();
+#pragma unroll
+                        for (int r = 0; r < R; ++r) {
+                            h[r] = ho[r];
+                            if (valid(r) && cs == 0) hsc[row(r)] = make_float4(ho[r] * rs, s[r], sgn2 * rs, 1.0f);
+                        }
+                        __syncthreads();
+                        phi_rows<MODE_PLAIN, true>(ib, rs, inv_n, unused, ksdp);
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r) phi[r] = 0.5f * (phi1[r] + phi[r]);
                 } else {
-                    if (want_ksd) phi_rows<MODE_PLAIN, true>(ib, rs, inv_n, phi, ksdp);
-                    else phi_rows<MODE_PLAIN, false>(ib, rs, inv_n, phi, ksdp);
+                    __syncthreads();  // previous transport finished reading hsc; mbuf complete
+                    gradient(P, beta, y_t, inv_s2, sq, rs, sgn2, mode);
+                    __syncthreads();  // hsc complete
+                    phi_dispatch(mode, want_ksd, ib, rs, inv_n, phi, ksdp);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) ho[r] = h[r];
                 }
                 // RMSProp + transport + Langevin noise (Jacobi: all phi from the iteration start)
 #pragma unroll
@@ -536,7 +624,7 @@ struct Engine {
                         diff = dstd * xi;
                     }
                     v[r] = vn;
-                    h[r] = clamp_logvol(h[r] + drift + diff);
+                    h[r] = clamp_logvol(ho[r] + drift + diff);
                 }
                 if (want_ksd) {
                     float kk[1] = {0.f};
@@ -656,20 +744,32 @@ struct Engine {
     }
 
     // Gradient for own rows; writes the hsc records of own rows.
+    // sq: the scale of the means in mbuf (sqrt(log2(e)/(2 sigma^2)) for the
+    // Gaussian state prior, 1/sqrt(nu_s sigma^2) for the Student-t one).
     SVPF_DEV void gradient(const svpf_fparams& P, float beta, float y_t, float inv_s2, float sq, float rs, float sgn2,
                            int mode) {
-        float sr[R], srd[R];
-        prior_sums(sq, sr, srd);
         const uint32_t F = P.flags;
+        const bool student = EXT && (F & SVPF_F_STUDENT_STATE) != 0;
+        float sr[R], srd[R], shp[R];
+        if (student) prior_sums_t(sq, 0.5f * (P.nu_state + 1.0f), sr, srd, shp);
+        else prior_sums(sq, sr, srd);
         const float nu = P.nu;
         const float eb = (F & SVPF_F_FAN) ? 1.0f : beta;
         const float ysq = y_t * y_t;
         const float log_y2 = logf(ysq + 1e-10f);
         const float isq = inv_s2 / sq;  // srd is in sq-scaled units
+        const float np1s = (P.nu_state + 1.0f) * sq;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            float gp = (srd[r] * isq) / (sr[r] + 1e-8f);
-            gp = -gp;
+            float gp, hpr;
+            if (student) {
+                const float w = 1.0f / (sr[r] + 1e-8f);
+                gp = -np1s * srd[r] * w;
+                hpr = -np1s * sq * shp[r] * w;
+            } else {
+                gp = -((srd[r] * isq) / (sr[r] + 1e-8f));
+                hpr = -inv_s2;
+            }
             float hj = h[r];
             float vol = safe_expf(hj);
             float A = ysq / (vol + 1e-8f) / nu;
@@ -683,7 +783,7 @@ struct Engine {
             float sj = g, cj = sgn2, cv = 1.0f;
             if (mode != MODE_PLAIN) {
                 float hl = -0.5f * (nu + 1.0f) * A / (opa * opa);
-                cv = fminf(fmaxf(-(hl - inv_s2), 0.1f), 100.0f);
+                cv = fminf(fmaxf(-(hl + hpr), 0.1f), 100.0f);
                 if (mode == MODE_NEWTON) {
                     sj = 0.7f * g / cv;
                     cj = sgn2 / cv;
@@ -709,6 +809,20 @@ struct Engine {
             phi[r] = p;
             if (KSD) ksdp[r] = s[r] * a1[r] + 2.0f * rs * (s[r] * a2[r] - a3[r]) + 2.0f * ib * a4[r];
             else ksdp[r] = 0.f;
+        }
+    }
+
+    SVPF_DEV void phi_dispatch(int mode, bool ksd, float ib, float rs, float inv_n, float (&phi)[R],
+                               float (&ksdp)[R]) const {
+        if (mode == MODE_NEWTON) {
+            if (ksd) phi_rows<MODE_NEWTON, true>(ib, rs, inv_n, phi, ksdp);
+            else phi_rows<MODE_NEWTON, false>(ib, rs, inv_n, phi, ksdp);
+        } else if (mode == MODE_FULL) {
+            if (ksd) phi_rows<MODE_FULL, true>(ib, rs, inv_n, phi, ksdp);
+            else phi_rows<MODE_FULL, false>(ib, rs, inv_n, phi, ksdp);
+        } else {
+            if (ksd) phi_rows<MODE_PLAIN, true>(ib, rs, inv_n, phi, ksdp);
+            else phi_rows<MODE_PLAIN, false>(ib, rs, inv_n, phi, ksdp);
         }
     }
 

@@ -20,7 +20,10 @@ enum {
     SVPF_TILE_R8_C1 = 5,
     SVPF_TILE_R2_C2 = 6,
     SVPF_TILE_R4_C2 = 7,
-    SVPF_NUM_TILES = 8
+    SVPF_NUM_TILES = 8,
+    // ORed into a tile id: the instantiation with the Heun step and the Student-t
+    // state, which filters using either need (without it they are rejected)
+    SVPF_TILE_EXT = 0x100
 };
 
 size_t svpf_lds_bytes(int nmax);

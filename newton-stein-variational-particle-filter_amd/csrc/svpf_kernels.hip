@@ -1,96 +1,37 @@
 // svpf_kernels.hip -- gfx950 kernels of the SVPF step.
 //
-// svpf_batched_kernel: a persistent workgroup pool pulls filters from an
-// atomic work queue (filters pre-sorted by decreasing N^2 cost) and advances
-// each one through ticks [t0, t1) with its particles resident in registers and
-// LDS.  A single filter is the B = 1 case.  Tile shape (rows per lane R,
-// column split CS) is chosen per filter on the host from the tile table.
+// svpf_batched_kernel (svpf_batched.hpp): a persistent workgroup pool advancing
+// filters tick by tick, one workgroup per filter.  The tile shape (rows per lane
+// R, column split CS) is chosen per filter on the host from the tile table; a
+// tile id with SVPF_TILE_EXT set selects the instantiation with the Heun step
+// and the Student-t state (built in svpf_kernels_ext.hip).
 #include <hip/hip_runtime.h>
 
-#include "svpf_engine.hpp"
-#include "svpf_kernels.h"
+#include "svpf_batched.hpp"
 
 using namespace svpf;
 
-namespace {
-
-template <int BS, int R, int CS>
-__device__ __forceinline__ void run_filter(char* lds, int nmax, const svpf_fparams* params, svpf_fstate* fstate,
-                           const svpf_fdesc* desc, int f, float* pool_h, float* pool_hp, float* pool_v, int t0,
-                           int t1) {
-    const svpf_fdesc* D = &desc[f];
-    const svpf_fparams* P = &params[D->param_idx];
-    FilterIO io{P, &fstate[f], D, pool_h + D->off, pool_hp + D->off, pool_v + D->off};
-    Engine<BS, R, CS> e;
-    e.bind(lds, P->n, nmax);
-    e.load(io);
-    for (int t = t0; t < t1; ++t) e.tick(io, t);
-    e.store(io);
-}
-
-}  // namespace
-
-template <int BS, int R, int CS>
-__global__ __launch_bounds__(BS) void svpf_batched_kernel(const svpf_fparams* __restrict__ params,
-                                                          svpf_fstate* __restrict__ fstate,
-                                                          const svpf_fdesc* __restrict__ desc,
-                                                          float* __restrict__ pool_h, float* __restrict__ pool_hp,
-                                                          float* __restrict__ pool_v, const int* __restrict__ order,
-                                                          int nf, int* __restrict__ queue, int nmax, int t0, int t1) {
-    extern __shared__ __align__(16) char lds[];
-    __shared__ int next;
-    if (queue == nullptr) {  // static assignment: workgroup b runs order[b]
-        if ((int)blockIdx.x < nf)
-            run_filter<BS, R, CS>(lds, nmax, params, fstate, desc, order[blockIdx.x], pool_h, pool_hp, pool_v, t0, t1);
-        return;
-    }
-    for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(queue, 1);
-        __syncthreads();
-        const int k = next;
-        __syncthreads();
-        if (k >= nf) break;  // every workgroup reaches this once the queue is drained
-        run_filter<BS, R, CS>(lds, nmax, params, fstate, desc, order[k], pool_h, pool_hp, pool_v, t0, t1);
-    }
+namespace svpf {
+template const void* batched_kernel<false>(int, int);
 }
 
 __global__ void svpf_queue_reset_kernel(int* queue) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *queue = 0;
 }
 
-#define SVPF_SET_LDS_ATTR(BSV, RV, CSV)                                                                      \
-    do {                                                                                                     \
-        hipError_t e_ = hipFuncSetAttribute((const void*)svpf_batched_kernel<BSV, RV, CSV>,                 \
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);               \
-        if (e_ != hipSuccess) return e_;                                                                     \
-    } while (0)
-
 // Lets every engine instance use up to 159 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
 extern "C" hipError_t svpf_kernels_init(void) {
     static hipError_t done = hipErrorNotReady;
     if (done == hipSuccess) return done;
     const int bytes = 160 * 1024 - 1024;
-    SVPF_SET_LDS_ATTR(256, 1, 4);
-    SVPF_SET_LDS_ATTR(256, 1, 2);
-    SVPF_SET_LDS_ATTR(256, 1, 1);
-    SVPF_SET_LDS_ATTR(256, 2, 1);
-    SVPF_SET_LDS_ATTR(256, 4, 1);
-    SVPF_SET_LDS_ATTR(256, 8, 1);
-    SVPF_SET_LDS_ATTR(256, 2, 2);
-    SVPF_SET_LDS_ATTR(256, 4, 2);
-    SVPF_SET_LDS_ATTR(512, 1, 4);
-    SVPF_SET_LDS_ATTR(512, 1, 2);
-    SVPF_SET_LDS_ATTR(512, 1, 1);
-    SVPF_SET_LDS_ATTR(512, 2, 1);
-    SVPF_SET_LDS_ATTR(512, 4, 1);
-    SVPF_SET_LDS_ATTR(512, 8, 1);
-    SVPF_SET_LDS_ATTR(512, 2, 2);
-    SVPF_SET_LDS_ATTR(512, 4, 2);
-    SVPF_SET_LDS_ATTR(1024, 1, 1);
-    SVPF_SET_LDS_ATTR(1024, 2, 1);
-    SVPF_SET_LDS_ATTR(1024, 4, 1);
-    SVPF_SET_LDS_ATTR(1024, 1, 2);
-    SVPF_SET_LDS_ATTR(1024, 1, 4);
+    for (int ext = 0; ext < 2; ++ext)
+        for (int bs = 256; bs <= 1024; bs *= 2)
+            for (int tile = 0; tile < SVPF_NUM_TILES; ++tile) {
+                const void* fn = ext ? batched_kernel<true>(bs, tile) : batched_kernel<false>(bs, tile);
+                if (!fn) continue;
+                hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+                if (e != hipSuccess) return e;
+            }
     done = hipSuccess;
     return done;
 }
@@ -118,48 +59,19 @@ extern "C" hipError_t svpf_launch_init(float* h, float* hp, float* v, int n, flo
 
 extern "C" size_t svpf_lds_bytes(int nmax) { return LdsLayout::bytes(nmax); }
 
-#define SVPF_OCC_TILE(BSV, RV, CSV) fn = (const void*)svpf_batched_kernel<BSV, RV, CSV>
+namespace {
+const void* batched_fn(int bs, int tile) {
+    if (tile & ~(SVPF_TILE_EXT | 0xff)) return nullptr;
+    const int t = tile & 0xff;
+    return (tile & SVPF_TILE_EXT) ? batched_kernel<true>(bs, t) : batched_kernel<false>(bs, t);
+}
+}  // namespace
 
 // Workgroups of the given tile resident per CU for filters of up to nmax particles (0: unknown tile).
 extern "C" int svpf_batched_wg_per_cu(int bs, int tile, int nmax) {
     if (svpf_kernels_init() != hipSuccess) return 0;
-    const void* fn = nullptr;
-    if (bs == 256) {
-        switch (tile) {
-            case SVPF_TILE_R1_C4: SVPF_OCC_TILE(256, 1, 4); break;
-            case SVPF_TILE_R1_C2: SVPF_OCC_TILE(256, 1, 2); break;
-            case SVPF_TILE_R1_C1: SVPF_OCC_TILE(256, 1, 1); break;
-            case SVPF_TILE_R2_C1: SVPF_OCC_TILE(256, 2, 1); break;
-            case SVPF_TILE_R4_C1: SVPF_OCC_TILE(256, 4, 1); break;
-            case SVPF_TILE_R8_C1: SVPF_OCC_TILE(256, 8, 1); break;
-            case SVPF_TILE_R2_C2: SVPF_OCC_TILE(256, 2, 2); break;
-            case SVPF_TILE_R4_C2: SVPF_OCC_TILE(256, 4, 2); break;
-            default: return 0;
-        }
-    } else if (bs == 512) {
-        switch (tile) {
-            case SVPF_TILE_R1_C4: SVPF_OCC_TILE(512, 1, 4); break;
-            case SVPF_TILE_R1_C2: SVPF_OCC_TILE(512, 1, 2); break;
-            case SVPF_TILE_R1_C1: SVPF_OCC_TILE(512, 1, 1); break;
-            case SVPF_TILE_R2_C1: SVPF_OCC_TILE(512, 2, 1); break;
-            case SVPF_TILE_R4_C1: SVPF_OCC_TILE(512, 4, 1); break;
-            case SVPF_TILE_R8_C1: SVPF_OCC_TILE(512, 8, 1); break;
-            case SVPF_TILE_R2_C2: SVPF_OCC_TILE(512, 2, 2); break;
-            case SVPF_TILE_R4_C2: SVPF_OCC_TILE(512, 4, 2); break;
-            default: return 0;
-        }
-    } else if (bs == 1024) {
-        switch (tile) {
-            case SVPF_TILE_R1_C4: SVPF_OCC_TILE(1024, 1, 4); break;
-            case SVPF_TILE_R1_C2: SVPF_OCC_TILE(1024, 1, 2); break;
-            case SVPF_TILE_R1_C1: SVPF_OCC_TILE(1024, 1, 1); break;
-            case SVPF_TILE_R2_C1: SVPF_OCC_TILE(1024, 2, 1); break;
-            case SVPF_TILE_R4_C1: SVPF_OCC_TILE(1024, 4, 1); break;
-            default: return 0;
-        }
-    } else {
-        return 0;
-    }
+    const void* fn = batched_fn(bs, tile);
+    if (!fn) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, bs, LdsLayout::bytes(nmax)) != hipSuccess) return 0;
     return nb;
@@ -167,6 +79,8 @@ extern "C" int svpf_batched_wg_per_cu(int bs, int tile, int nmax) {
 
 extern "C" int svpf_tile_capacity(int tile, int bs) {
     if (bs != 256 && bs != 512 && bs != 1024) return 0;
+    if (tile & ~(SVPF_TILE_EXT | 0xff)) return 0;
+    tile &= 0xff;
     if (bs == 1024 && tile > SVPF_TILE_R4_C1) return 0;  // not instantiated at 1024 threads
     switch (tile) {
         case SVPF_TILE_R1_C4: return bs / 4;
@@ -181,52 +95,17 @@ extern "C" int svpf_tile_capacity(int tile, int bs) {
     }
 }
 
-#define SVPF_LAUNCH_TILE(BSV, RV, CSV)                                                                   \
-    svpf_batched_kernel<BSV, RV, CSV><<<grid, BSV, lds, stream>>>(params, fstate, desc, pool_h, pool_hp, pool_v, \
-                                                                 order, nf, queue, nmax, t0, t1)
-
 extern "C" hipError_t svpf_launch_batched(int bs, int tile, int grid, const svpf_fparams* params,
                                           svpf_fstate* fstate, const svpf_fdesc* desc, float* pool_h,
                                           float* pool_hp, float* pool_v, const int* order, int nf, int* queue,
                                           int nmax, int t0, int t1, hipStream_t stream) {
+    const void* fn = batched_fn(bs, tile);
+    if (!fn || grid <= 0) return hipErrorInvalidValue;
     size_t lds = LdsLayout::bytes(nmax);
     if (queue) svpf_queue_reset_kernel<<<1, 64, 0, stream>>>(queue);
-    if (bs == 256) {
-        switch (tile) {
-            case SVPF_TILE_R1_C4: SVPF_LAUNCH_TILE(256, 1, 4); break;
-            case SVPF_TILE_R1_C2: SVPF_LAUNCH_TILE(256, 1, 2); break;
-            case SVPF_TILE_R1_C1: SVPF_LAUNCH_TILE(256, 1, 1); break;
-            case SVPF_TILE_R2_C1: SVPF_LAUNCH_TILE(256, 2, 1); break;
-            case SVPF_TILE_R4_C1: SVPF_LAUNCH_TILE(256, 4, 1); break;
-            case SVPF_TILE_R8_C1: SVPF_LAUNCH_TILE(256, 8, 1); break;
-            case SVPF_TILE_R2_C2: SVPF_LAUNCH_TILE(256, 2, 2); break;
-            case SVPF_TILE_R4_C2: SVPF_LAUNCH_TILE(256, 4, 2); break;
-            default: return hipErrorInvalidValue;
-        }
-    } else if (bs == 512) {
-        switch (tile) {
-            case SVPF_TILE_R1_C4: SVPF_LAUNCH_TILE(512, 1, 4); break;
-            case SVPF_TILE_R1_C2: SVPF_LAUNCH_TILE(512, 1, 2); break;
-            case SVPF_TILE_R1_C1: SVPF_LAUNCH_TILE(512, 1, 1); break;
-            case SVPF_TILE_R2_C1: SVPF_LAUNCH_TILE(512, 2, 1); break;
-            case SVPF_TILE_R4_C1: SVPF_LAUNCH_TILE(512, 4, 1); break;
-            case SVPF_TILE_R8_C1: SVPF_LAUNCH_TILE(512, 8, 1); break;
-            case SVPF_TILE_R2_C2: SVPF_LAUNCH_TILE(512, 2, 2); break;
-            case SVPF_TILE_R4_C2: SVPF_LAUNCH_TILE(512, 4, 2); break;
-            default: return hipErrorInvalidValue;
-        }
-    } else if (bs == 1024) {
-        switch (tile) {
-            case SVPF_TILE_R1_C4: SVPF_LAUNCH_TILE(1024, 1, 4); break;
-            case SVPF_TILE_R1_C2: SVPF_LAUNCH_TILE(1024, 1, 2); break;
-            case SVPF_TILE_R1_C1: SVPF_LAUNCH_TILE(1024, 1, 1); break;
-            case SVPF_TILE_R2_C1: SVPF_LAUNCH_TILE(1024, 2, 1); break;
-            case SVPF_TILE_R4_C1: SVPF_LAUNCH_TILE(1024, 4, 1); break;
-            default: return hipErrorInvalidValue;
-        }
-    } else {
-        return hipErrorInvalidValue;
-    }
+    void* args[] = {&params, &fstate, &desc, &pool_h, &pool_hp, &pool_v, &order, &nf, &queue, &nmax, &t0, &t1};
+    hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(bs), args, lds, stream);
+    if (e != hipSuccess) return e;
     return hipGetLastError();
 }
 

@@ -37,6 +37,8 @@ enum {
     SVPF_F_ADAPT_SIGMA = 1u << 17,
     SVPF_F_KSD_EARLY_STOP = 1u << 18,
     SVPF_F_PAPER_SIGN = 1u << 19,
+    SVPF_F_HEUN = 1u << 20,           // Heun predictor-corrector Stein step
+    SVPF_F_STUDENT_STATE = 1u << 21,  // Student-t state noise and transition prior (nu_state)
 };
 
 // Bandwidth rules.
@@ -60,7 +62,8 @@ typedef struct svpf_fparams {
     float mu_q, mu_rscale, mu_min, mu_max;
     float sb_thr, sb_max, ksd_thr, st_const;
     float implied_offset, base_step;
-    float pad1[22];
+    float nu_state;
+    float pad1[21];
 } svpf_fparams;
 
 // Per-filter scalar state carried between ticks (host fields of SVPFState and
@@ -106,7 +109,9 @@ typedef struct svpf_fdesc {
 } svpf_fdesc;
 
 // Noise purposes: third word of the counter-generator counter.
-enum { SVPF_PURPOSE_PREDICT = 0, SVPF_PURPOSE_REJUV = 1, SVPF_PURPOSE_INIT = 255, SVPF_PURPOSE_SVLD0 = 16 };
+// SVPF_PURPOSE_STATE_T0 + k: normal k of the chi-squared sum of a Student-t state draw.
+enum { SVPF_PURPOSE_PREDICT = 0, SVPF_PURPOSE_REJUV = 1, SVPF_PURPOSE_INIT = 255, SVPF_PURPOSE_SVLD0 = 16,
+       SVPF_PURPOSE_STATE_T0 = 128 };
 
 // ---------------------------------------------------------------------------
 // Differentiable form: B filters of n particles with the plain feature set

@@ -21,6 +21,8 @@ F_EXACT_GRAD, F_NEWTON, F_FULL_NEWTON, F_SVLD, F_ANNEAL, F_ADAPT_BETA, F_LINEAR_
 (F_ASYM_RHO, F_LOCAL_PARAMS, F_GUIDED, F_GUIDE, F_GUIDE_PRESERVING, F_ADAPT_GUIDE, F_REJUV,
  F_FAN) = (1 << k for k in range(8, 16))
 F_ADAPT_MU, F_ADAPT_SIGMA, F_KSD_EARLY_STOP, F_PAPER_SIGN = (1 << k for k in range(16, 20))
+F_HEUN, F_STUDENT_STATE = 1 << 20, 1 << 21
+TILE_EXT = 0x100  # ORed into a tile id: the kernels with the Heun step and the Student-t state
 
 _PARAM_FLOATS = ("mu", "rho", "sigma_z", "gamma", "nu", "lik_offset", "temperature", "rms_rho", "rms_eps", "mim_p",
                  "mim_scale", "rho_up", "rho_down", "delta_rho", "delta_sigma", "ga_base", "ga_shock", "ga_thr",
@@ -28,7 +30,7 @@ _PARAM_FLOATS = ("mu", "rho", "sigma_z", "gamma", "nu", "lik_offset", "temperatu
                  "mu_max", "sb_thr", "sb_max", "ksd_thr", "st_const", "implied_offset", "base_step")
 FPARAMS_DTYPE = np.dtype([("n", "<i4"), ("flags", "<u4"), ("n_anneal", "<i4"), ("stein_min", "<i4"),
                           ("stein_max", "<i4"), ("bandwidth_mode", "<i4"), ("pad0", "<i4", (2,))]
-                         + [(f, "<f4") for f in _PARAM_FLOATS] + [("pad1", "<f4", (22,))])
+                         + [(f, "<f4") for f in _PARAM_FLOATS] + [("nu_state", "<f4"), ("pad1", "<f4", (21,))])
 FSTATE_DTYPE = np.dtype([("timestep", "<i4"), ("guide_initialized", "<i4"), ("stein_steps_used", "<i4"),
                          ("status", "<i4")]
                         + [(f, "<f4") for f in ("ksd_prev", "vol_prev", "h_mean_prev", "guide_mean", "guide_var",
@@ -98,7 +100,8 @@ def pack_params(cfg: dict) -> np.ndarray:
             ("use_guided", F_GUIDED), ("use_guide", F_GUIDE), ("use_guide_preserving", F_GUIDE_PRESERVING),
             ("use_adaptive_guide", F_ADAPT_GUIDE), ("use_rejuvenation", F_REJUV), ("use_fan_mode", F_FAN),
             ("use_adaptive_mu", F_ADAPT_MU), ("use_adaptive_sigma", F_ADAPT_SIGMA),
-            ("use_ksd_early_stop", F_KSD_EARLY_STOP)]
+            ("use_ksd_early_stop", F_KSD_EARLY_STOP), ("use_heun", F_HEUN),
+            ("use_student_t_state", F_STUDENT_STATE)]
     for k, b in bits:
         if cfg[k]:
             f |= b
@@ -108,6 +111,7 @@ def pack_params(cfg: dict) -> np.ndarray:
         f |= F_PAPER_SIGN
     p["n"] = cfg["n"]
     p["flags"] = f
+    p["nu_state"] = max(float(cfg["nu_state"]), 2.5)
     p["n_anneal"] = cfg["n_anneal_steps"]
     p["stein_min"] = cfg["stein_min_steps"]
     p["stein_max"] = cfg["stein_max_steps"]
@@ -129,6 +133,16 @@ def pack_params(cfg: dict) -> np.ndarray:
     return p
 
 
+def stationary_var(cfg: dict) -> float:
+    """Stationary variance of the state, scaled by nu_s / (nu_s - 2) for the
+    Student-t state with nu_s clamped to >= 2.5 (src/svpf_optimized_graph.cu:366-383)."""
+    v = cfg["sigma_z"] ** 2 / (1.0 - cfg["rho"] ** 2 + 1e-6)
+    if cfg.get("use_student_t_state"):
+        nu_s = max(float(cfg["nu_state"]), 2.5)
+        v *= nu_s / (nu_s - 2.0)
+    return v
+
+
 def initial_fstate(cfg: dict) -> np.ndarray:
     """svpf_fstate as svpf_create + svpf_initialize leave it (src/svpf_optimized_graph.cu:349-451)."""
     s = np.zeros(1, dtype=FSTATE_DTYPE)[0]
@@ -136,7 +150,7 @@ def initial_fstate(cfg: dict) -> np.ndarray:
     s["vol_prev"] = 0.05
     s["h_mean_prev"] = -3.5
     s["guide_mean"] = cfg["mu"]
-    s["guide_var"] = cfg["sigma_z"] ** 2 / (1.0 - cfg["rho"] ** 2 + 1e-6)
+    s["guide_var"] = stationary_var(cfg)
     s["mu_state"] = cfg["mu"]
     s["mu_var"] = 1.0
     s["sigma_z_eff"] = cfg["sigma_z"]
@@ -184,7 +198,9 @@ class Batch:
         desc = np.zeros(B, dtype=FDESC_DTYPE)
         cost = self.n.astype(np.float64) ** 2
         table = tuned_table() if tile_table is None else tile_table
-        tiles = [choose_tile(int(nb), table) for nb in self.n]
+        # filters with the Heun step or the Student-t state run in the extended instantiation
+        tiles = [(lambda bt, c: (bt[0], bt[1] | (TILE_EXT if c["use_heun"] or c["use_student_t_state"] else 0)))(
+            choose_tile(int(nb), table), c) for nb, c in zip(self.n, cfgs)]
         base = self.pool.data_ptr()
         for b in range(B):
             d = desc[b]

@@ -7,6 +7,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from .batched import stationary_var
 from .config import implied_offset, student_t_const
 
 # config-dict field -> SVPFState field (identity unless listed)
@@ -112,7 +113,8 @@ class Filter:
         st.y_prev = 0.0
         st.guide_initialized = 0
         st.guide_mean = c["mu"]
-        st.guide_var = c["sigma_z"] ** 2 / (1.0 - c["rho"] ** 2 + 1e-6)
+        st.guide_var = stationary_var(c)
+        st.nu_state = max(float(c.get("nu_state", 5.0)), 2.5)
         st.ksd_prev = 1e10
         st.vol_prev = 0.05
         st.mu_state = c["mu"]

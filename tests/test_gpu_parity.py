@@ -197,10 +197,10 @@ def test_unsupported_and_uninitialized_fail_loudly(torch_cuda):
     with pytest.raises(RuntimeError, match="not initialized"):
         f.step(0.1, 0.0)
     f.initialize(7)
-    f.state.use_heun = 1
+    f.state.use_two_factor = 1  # the two-factor form is the svpf_diff2d_* entry points
     with pytest.raises(RuntimeError, match="unsupported"):
         f.step(0.1, 0.0)
-    f.state.use_heun = 0
+    f.state.use_two_factor = 0
     f.step(0.1, 0.0)
     assert f.L.svpf_last_error(f.sp) == _lib.SVPF_OK
     assert not f.L.svpf_create(0, 5, 5.0, None)
