@@ -1,4 +1,5 @@
-"""ctypes binding of libsvpf_amd.so (the C ABI declared in include/svpf.h).
+"""ctypes binding of libsvpf_amd.so (the C ABI declared in include/svpf.h and
+include/svpf_fully_fused_v2.h).
 
 The library is the product path: every call here runs the HIP kernels.  There
 is no CPU fallback -- loading fails loudly when the library is missing.
@@ -76,7 +77,16 @@ SVPF_ERR_NOT_INITIALIZED = 4
 SVPF_ERR_OUT_OF_MEMORY = 5
 SVPF_ERR_TIMEOUT = 6
 
-# Every symbol include/svpf.h declares (non-inline).
+
+
+class SVPFV2Noise(ctypes.Structure):
+    """svpf_v2_noise of include/svpf_fully_fused_v2.h."""
+    _fields_ = [("seed", ctypes.c_ulonglong), ("stream_id", ctypes.c_uint), ("diffusion_rows", c_int),
+                ("z_state", c_void_p), ("u_jump", c_void_p), ("z_diffusion", c_void_p), ("u_rejuv", c_void_p),
+                ("z_rejuv", c_void_p)]
+
+
+# Every symbol include/svpf.h and include/svpf_fully_fused_v2.h declare (non-inline).
 EXPORTED = (
     "svpf_create", "svpf_destroy", "svpf_initialize", "svpf_step", "svpf_step_seeded", "svpf_run_sequence",
     "svpf_run_sequence_device", "svpf_step_adaptive", "svpf_step_graph", "svpf_graph_is_captured",
@@ -86,7 +96,26 @@ EXPORTED = (
     "svpf_lds_bytes_query", "svpf_version", "svpf_set_coop_shape", "svpf_get_coop_shape",
     "svpf_last_error_string", "svpf_diff_tape_floats", "svpf_diff_forward", "svpf_diff_backward",
     "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
-    "svpf_set_grid_path", "svpf_get_path",
+    "svpf_set_grid_path", "svpf_get_path", "svpf_fully_fused_step_v2", "svpf_fully_fused_v2_supported",
+)
+
+# svpf_fully_fused_step_v2 argument kinds, in order (include/svpf_fully_fused_v2.h)
+V2_ARGS = (
+    [("h", "p"), ("h_prev", "p"), ("grad_log_p", "p"), ("log_weights", "p"), ("d_grad_v", "p"),
+     ("d_inv_hessian", "p"), ("noise", "noise"), ("y_t", "f"), ("y_prev", "f"), ("h_mean_prev", "f"),
+     ("vol_prev", "f"), ("ksd_prev", "f"), ("d_h_mean", "p"), ("d_vol", "p"), ("d_loglik", "p"),
+     ("d_bandwidth", "p"), ("d_ksd", "p"), ("d_guide_mean", "p"), ("d_guide_var", "p"), ("rho", "f"),
+     ("sigma_z", "f"), ("mu", "f"), ("nu", "f"), ("lik_offset", "f"), ("student_t_const", "f"),
+     ("student_t_implied_offset", "f"), ("gamma", "f"), ("delta_rho", "f"), ("delta_sigma", "f"),
+     ("mim_jump_prob", "f"), ("mim_jump_scale", "f"), ("guide_strength_base", "f"), ("guide_strength_max", "f"),
+     ("guide_innovation_threshold", "f"), ("guided_alpha_base", "f"), ("guided_alpha_shock", "f"),
+     ("guided_innovation_thresh_predict", "f"), ("use_guide", "i"), ("use_guided_predict", "i"),
+     ("use_guide_preserving", "i"), ("use_newton", "i"), ("use_full_newton", "i"), ("use_rejuvenation", "i"),
+     ("rejuv_ksd_threshold", "f"), ("rejuv_prob", "f"), ("rejuv_blend", "f"), ("use_adaptive_mu", "i"),
+     ("d_mu_state", "p"), ("mu_ema_alpha", "f"), ("use_adaptive_sigma", "i"), ("sigma_boost_threshold", "f"),
+     ("sigma_boost_max", "f"), ("step_size", "f"), ("temperature", "f"), ("rmsprop_rho", "f"),
+     ("rmsprop_eps", "f"), ("n_stein_steps", "i"), ("n_anneal_steps", "i"), ("stein_sign_mode", "i"), ("n", "i"),
+     ("timestep", "i"), ("stream", "p")]
 )
 
 _lib = None
@@ -165,6 +194,11 @@ def load(path: str | None = None):
     L.svpf_get_path.restype = c_int
     L.svpf_get_path.argtypes = [SP]
     L.svpf_get_coop_shape.argtypes = [SP, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
+    kinds = {"p": c_void_p, "f": c_float, "i": c_int, "noise": ctypes.POINTER(SVPFV2Noise)}
+    L.svpf_fully_fused_step_v2.restype = c_int
+    L.svpf_fully_fused_step_v2.argtypes = [kinds[k] for _, k in V2_ARGS]
+    L.svpf_fully_fused_v2_supported.restype = ctypes.c_bool
+    L.svpf_fully_fused_v2_supported.argtypes = [c_int]
     L.svpf_version.restype = ctypes.c_char_p
     L.svpf_version.argtypes = []
     if path is None:

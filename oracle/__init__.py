@@ -60,6 +60,28 @@ class Noise(ctypes.Structure):
     ]
 
 
+V2_INT_FIELDS = ("n", "timestep", "use_guide", "use_guided_predict", "use_guide_preserving", "use_newton",
+                 "use_full_newton", "use_rejuvenation", "use_adaptive_mu", "use_adaptive_sigma", "n_stein_steps",
+                 "n_anneal_steps", "stein_sign_mode")
+V2_DOUBLE_FIELDS = ("y_t", "y_prev", "h_mean_prev", "vol_prev", "ksd_prev", "rho", "sigma_z", "mu", "nu",
+                    "lik_offset", "student_t_const", "student_t_implied_offset", "gamma", "delta_rho", "delta_sigma",
+                    "mim_jump_prob", "mim_jump_scale", "guide_strength_base", "guide_strength_max",
+                    "guide_innovation_threshold", "guided_alpha_base", "guided_alpha_shock",
+                    "guided_innovation_thresh_predict", "rejuv_ksd_threshold", "rejuv_prob", "rejuv_blend",
+                    "mu_ema_alpha", "sigma_boost_threshold", "sigma_boost_max", "step_size", "temperature",
+                    "rmsprop_rho", "rmsprop_eps")
+
+
+class V2Args(ctypes.Structure):
+    _fields_ = ([(f, ctypes.c_int) for f in V2_INT_FIELDS] + [("pad_", ctypes.c_int)]
+                + [(f, ctypes.c_double) for f in V2_DOUBLE_FIELDS])
+
+
+class V2IO(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_double) for f in ("guide_mean", "guide_var", "mu_state", "h_mean", "vol", "loglik",
+                                                "bandwidth", "ksd")]
+
+
 class Out(ctypes.Structure):
     _fields_ = [
         ("loglik", ctypes.c_double), ("pred_loglik", ctypes.c_double), ("vol", ctypes.c_double),
@@ -111,6 +133,8 @@ def lib():
         L.svpfo_u01_from_bits.argtypes = [ctypes.c_uint32]
         L.svpfo_normal_from_bits.restype = D
         L.svpfo_normal_from_bits.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.svpfo_v2_step.restype = I
+        L.svpfo_v2_step.argtypes = [P, P, P, P, P, P, P, P, P]
         L.svpfo_student_t_draw.restype = D
         L.svpfo_student_t_draw.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, D]
         _lib = L
@@ -263,3 +287,49 @@ class OracleFilter:
             raise RuntimeError("oracle run failed")
         outs["steps"] = steps
         return outs
+
+
+def v2_step(args: dict, h, h_prev, v, io: dict, noise=None, seed=0, filter_id=0):
+    """One call of the single-block fused step (svpfo_v2_step).  args: the
+    scalar arguments (V2_INT_FIELDS, V2_DOUBLE_FIELDS); h, h_prev, v: float64
+    arrays updated in place; io: guide_mean, guide_var, mu_state (read and
+    updated).  noise: dict of float32 arrays z_pred [n], u_pred [n], xi
+    [rows, n], u_rej [n], z_rej [n], or None for the counter generator keyed
+    by (seed, filter_id, timestep).  Returns (grad, log_w, inv_hess, io)."""
+    a = V2Args()
+    for f in V2_INT_FIELDS:
+        setattr(a, f, int(args[f]))
+    for f in V2_DOUBLE_FIELDS:
+        setattr(a, f, float(args[f]))
+    n = a.n
+    keep = []
+
+    def f32(x):
+        if x is None:
+            return None
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        keep.append(x)
+        return x
+
+    nz = Noise()
+    noise = noise or {}
+    xa = f32(noise.get("xi"))
+    nz.z_pred = _ptr(f32(noise.get("z_pred")))
+    nz.u_pred = _ptr(f32(noise.get("u_pred")))
+    nz.xi = _ptr(xa)
+    nz.xi_rows = 0 if xa is None else xa.size // n
+    nz.u_rej = _ptr(f32(noise.get("u_rej")))
+    nz.z_rej = _ptr(f32(noise.get("z_rej")))
+    nz.seed = seed
+    nz.filter_id = filter_id
+    st = V2IO()
+    for k in ("guide_mean", "guide_var", "mu_state"):
+        setattr(st, k, float(io.get(k, 0.0)))
+    grad, log_w, inv_hess = np.zeros(n), np.zeros(n), np.zeros(n)
+    for arr in (h, h_prev, v):
+        assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"] and arr.size == n
+    rc = lib().svpfo_v2_step(ctypes.byref(a), _ptr(h), _ptr(h_prev), _ptr(grad), _ptr(log_w), _ptr(v),
+                             _ptr(inv_hess), ctypes.byref(nz), ctypes.byref(st))
+    if rc:
+        raise RuntimeError("oracle v2 step failed")
+    return grad, log_w, inv_hess, {f: getattr(st, f) for f, _ in V2IO._fields_}

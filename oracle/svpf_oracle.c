@@ -10,6 +10,7 @@
  *   src/svpf_optimized_graph.cu:915-1389  svpf_step_graph orchestration
  *   src/svpf_opt_kernels.cu               predict / guide / gradient / transport / KSD / outputs
  *   include/svpf_kernels.cuh:509-541      host EKF guide
+ *   src/svpf_fully_fused_v2.cu:90-712     the single-block fused step (svpfo_v2_step)
  *
  * Semantics fixed here where the reference is ambiguous:
  *   - Stein transport is a Jacobi update: every phi_i uses the positions of
@@ -704,5 +705,245 @@ int svpfo_run(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, do
         if (steps) steps[t] = o.steps;
         yp = (double)y[t];
     }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The single-block fused step, src/svpf_fully_fused_v2.cu:90-712            */
+/* ------------------------------------------------------------------------ */
+/* Its semantics differ from svpf_step_graph's: the EKF guide uses the
+ * adaptive mu and re-initialises when guide_var < 1e-6 (:180-263); local
+ * parameters apply whenever delta_rho or delta_sigma is positive and MIM
+ * whenever the jump draw falls under mim_jump_prob (:282-327); the bandwidth is
+ * Silverman's on a two-pass variance (:397-424); beta ramps 0.3 + 0.5 ai/(A-1)
+ * with the last stage at 1 and the remainder iterations in the last stage
+ * (:430-455); the prior gradient is clamped to +-10 on its own (:466-484); full
+ * Newton is selected by use_full_newton alone, and use_newton scales the
+ * repulsion by the stored per-particle Hessian term (:534-574); the KSD of the
+ * last iteration takes raw scores, undoing the Newton preconditioning
+ * (:579-596); rejuvenation does not need the guide (:640-652); adaptive mu is
+ * an EMA clamped to [-10, 0] (:700-706). */
+static double v2_normal(const svpfo_noise* nz, const float* arr, int i, uint32_t tick, uint32_t purpose,
+                        double missing) {
+    if (nz->z_pred == NULL) return noise_normal(nz, NULL, i, tick, purpose);
+    return arr ? (double)arr[i] : missing;
+}
+static double v2_uniform(const svpfo_noise* nz, const float* arr, int i, uint32_t tick, uint32_t purpose,
+                         double missing) {
+    if (nz->z_pred == NULL) return noise_uniform(nz, NULL, i, tick, purpose);
+    return arr ? (double)arr[i] : missing;
+}
+
+int svpfo_v2_step(const svpfo_v2_args* a, double* h, double* h_prev, double* grad, double* log_w, double* v,
+                  double* inv_hess, const svpfo_noise* nz, svpfo_v2_io* io) {
+    const int n = a->n;
+    if (n < 1) return 1;
+    const uint32_t tick = (uint32_t)a->timestep;
+    const double rho = a->rho, nu = a->nu;
+
+    /* phase 0: scalars (:180-263) */
+    const double mu_eff = a->use_adaptive_mu ? io->mu_state : a->mu;
+    double gm_prev = io->guide_mean, gvar = io->guide_var;
+    if (gvar < 1e-6) {
+        gm_prev = a->mu;
+        gvar = a->sigma_z * a->sigma_z / (1.0 - rho * rho);
+    }
+    double boost = 1.0;
+    if (a->use_adaptive_sigma && a->timestep > 0) {
+        double rz = fabs(a->y_t) / (a->vol_prev > 1e-4 ? a->vol_prev : 1e-4);
+        if (rz > a->sigma_boost_threshold) {
+            double sev = (rz - a->sigma_boost_threshold) / 3.0;
+            boost = 1.0 + (a->sigma_boost_max - 1.0) * (sev < 1.0 ? sev : 1.0);
+        }
+    }
+    const double sig = a->sigma_z * boost;
+    double gmean, gstr;
+    if (a->use_guide) {
+        double m_pred = mu_eff + rho * (gm_prev - mu_eff);
+        double P_pred = rho * rho * gvar + a->sigma_z * a->sigma_z;
+        double Sv = P_pred + (4.93 + 2.0);
+        double K = P_pred / (Sv + 1e-8);
+        double innov = log(a->y_t * a->y_t + 1e-8) - (m_pred - a->student_t_implied_offset);
+        gmean = m_pred + K * innov;
+        gvar = (1.0 - K) * P_pred;
+        io->guide_mean = gmean;
+        io->guide_var = gvar;
+        double rz = fabs(a->y_t) / (a->vol_prev > 1e-4 ? a->vol_prev : 1e-4);
+        gstr = a->guide_strength_base;
+        if (rz > a->guide_innovation_threshold) {
+            double sev = (rz - a->guide_innovation_threshold) / 3.0;
+            gstr += (a->guide_strength_max - a->guide_strength_base) * (sev < 1.0 ? sev : 1.0);
+        }
+    } else {
+        gmean = a->h_mean_prev;
+        gvar = 1.0;
+        gstr = 0.0;
+    }
+
+    /* phase 1: mixture means from the previous posterior, predict (:270-370) */
+    double* m = (double*)malloc((size_t)n * sizeof(double));
+    double* s = (double*)malloc((size_t)n * sizeof(double));
+    double* hs = (double*)malloc((size_t)n * sizeof(double));
+    double* hn = (double*)malloc((size_t)n * sizeof(double));
+    double* kp = (double*)calloc((size_t)n, sizeof(double));
+    for (int k = 0; k < n; ++k)
+        m[k] = mu_eff + rho * (h[k] - mu_eff) + a->gamma * a->y_prev / (exp(h[k] * 0.5) + 1e-8);
+    const int local = a->delta_rho > 0.0 || a->delta_sigma > 0.0;
+    for (int i = 0; i < n; ++i) {
+        double ho = h[i], lr = rho, ls = sig;
+        if (local) {
+            double dev = ho - a->h_mean_prev;
+            lr = clamp_d(rho + a->delta_rho * tanh(dev), 0.0, 0.999);
+            ls = sig * (1.0 + a->delta_sigma * fabs(dev));
+        }
+        double lev = fabs(a->gamma) > 1e-6 ? a->gamma * a->y_prev / (exp(ho * 0.5) + 1e-8) : 0.0;
+        double pm = mu_eff + lr * (ho - mu_eff) + lev;
+        double z = v2_normal(nz, nz->z_pred, i, tick, PURPOSE_PREDICT, 0.0);
+        double u = v2_uniform(nz, nz->u_pred, i, tick, PURPOSE_PREDICT, 1.0);
+        double ms = ls * (1.0 + (u < a->mim_jump_prob ? 1.0 : 0.0) * (a->mim_jump_scale - 1.0));
+        if (a->use_guided_predict) {
+            double mi = log(a->y_t * a->y_t + 1e-10) + a->student_t_implied_offset;
+            if (mi < -5.0) mi = -5.0;
+            double zs = (mi - pm) / 2.5;
+            double act = zs > a->guided_innovation_thresh_predict ? tanh(zs - a->guided_innovation_thresh_predict) : 0.0;
+            double al = a->guided_alpha_base + (a->guided_alpha_shock - a->guided_alpha_base) * act;
+            pm = (1.0 - al) * pm + al * mi;
+        }
+        h_prev[i] = ho;
+        h[i] = clamp_logvol(pm + ms * z);
+    }
+
+    /* phase 2: guide (:372-390) */
+    if (a->use_guide && gstr > 0.0) {
+        for (int i = 0; i < n; ++i) {
+            double x = a->use_guide_preserving ? h[i] + gstr * (gmean - a->h_mean_prev)
+                                               : (1.0 - gstr) * h[i] + gstr * gmean;
+            h[i] = clamp_logvol(x);
+        }
+    }
+
+    /* phase 3: Silverman bandwidth on the two-pass variance (:397-424) */
+    double mean = 0.0, var = 0.0;
+    for (int i = 0; i < n; ++i) mean += h[i];
+    mean /= n;
+    for (int i = 0; i < n; ++i) var += (h[i] - mean) * (h[i] - mean);
+    var /= n;
+    double bw = clamp_d(1.06 * sqrt(var > 1e-6 ? var : 1e-6) * pow((double)n, -0.2), 0.05, 2.0);
+    io->bandwidth = bw;
+    const double ib = 1.0 / (bw * bw);
+
+    /* phase 4: Stein loop (:430-630) */
+    const int na = a->n_anneal_steps < 1 ? 1 : a->n_anneal_steps;
+    const int ns = a->n_stein_steps < 1 ? 1 : a->n_stein_steps;
+    const int spa = ns / na, rem = ns - spa * na;
+    const double sgn = a->stein_sign_mode == 1 ? 1.0 : -1.0;
+    const double inv_s2 = 1.0 / (sig * sig), i2s2 = 0.5 * inv_s2;
+    int step = 0;
+    for (int ai = 0; ai < na; ++ai) {
+        double beta = (na <= 1 || ai == na - 1) ? 1.0 : 0.3 + 0.5 * (double)ai / (double)(na - 1);
+        int si = spa + (ai == na - 1 ? rem : 0);
+        for (int it = 0; it < si; ++it) {
+            ++step;
+            const int last = step == ns;
+            for (int j = 0; j < n; ++j) {
+                double hj = h[j], lmax = -1e10;
+                for (int k = 0; k < n; ++k) {
+                    double d = hj - m[k], l = -d * d * i2s2;
+                    if (l > lmax) lmax = l;
+                }
+                double sr = 0.0, wg = 0.0;
+                for (int k = 0; k < n; ++k) {
+                    double d = hj - m[k], r = exp(-d * d * i2s2 - lmax);
+                    sr += r;
+                    wg -= r * d * inv_s2;
+                }
+                double gp = clamp_d(wg / (sr + 1e-8), -10.0, 10.0);
+                double vol = exp(hj * 0.5);
+                double A = a->y_t * a->y_t / (vol * vol + 1e-8) / nu, opa = 1.0 + A;
+                log_w[j] = a->student_t_const - 0.5 * hj - 0.5 * (nu + 1.0) * log(opa);
+                double gl = -0.5 + 0.5 * (nu + 1.0) * A / opa - a->lik_offset;
+                double g = clamp_d(gp + beta * gl, -10.0, 10.0);
+                grad[j] = g;
+                double cv = clamp_d(-(-0.5 * (nu + 1.0) * A / (opa * opa) - inv_s2), 0.1, 100.0);
+                if (a->use_newton || a->use_full_newton) inv_hess[j] = cv;
+                if (a->use_full_newton) {
+                    s[j] = g;
+                    hs[j] = cv;
+                } else if (a->use_newton) {
+                    s[j] = 0.7 * g / cv;
+                    hs[j] = 1.0 / cv;
+                } else {
+                    s[j] = g;
+                    hs[j] = 1.0;
+                }
+            }
+            const double eff = a->step_size * sqrt(beta);
+            for (int i = 0; i < n; ++i) {
+                double ks = 0.0, gk = 0.0, hw = 0.0, kn = 0.0, kq = 0.0;
+                for (int j = 0; j < n; ++j) {
+                    double d = h[i] - h[j], dist = d * d * ib, K = 1.0 / (1.0 + dist), K2 = K * K;
+                    ks += K * s[j];
+                    gk += sgn * 2.0 * d * ib * K2 * (a->use_newton ? hs[j] : 1.0);
+                    if (a->use_full_newton) {
+                        hw += hs[j] * K + 2.0 * ib * K2 * fabs(3.0 * dist - 1.0);
+                        kn += K;
+                    }
+                    if (last) {
+                        double ri = grad[i], rj = a->use_newton ? s[j] / (0.7 * hs[j] + 1e-8) : s[j];
+                        double gx = -2.0 * d * ib * K2;
+                        kq += K * ri * rj - ri * gx + rj * gx + 2.0 * ib * K2 * (4.0 * dist * K - 1.0);
+                    }
+                }
+                double phi = (ks + gk) / n;
+                if (a->use_full_newton) phi = phi / clamp_d(hw / (kn > 1e-6 ? kn : 1e-6), 0.1, 100.0) * 0.7;
+                double vn = a->rmsprop_rho * v[i] + (1.0 - a->rmsprop_rho) * phi * phi;
+                v[i] = vn;
+                double diff = 0.0;
+                if (a->temperature > 1e-6) {
+                    double xi;
+                    if (nz->z_pred == NULL) xi = noise_normal(nz, NULL, i, tick, PURPOSE_SVLD0 + (uint32_t)(step - 1));
+                    else xi = (nz->xi && step - 1 < nz->xi_rows) ? (double)nz->xi[(size_t)(step - 1) * n + i] : 0.0;
+                    diff = sqrt(2.0 * eff * a->temperature) * xi;
+                }
+                hn[i] = clamp_logvol(h[i] + eff * phi / sqrt(vn + a->rmsprop_eps) + diff);
+                if (last) kp[i] = kq;
+            }
+            memcpy(h, hn, (size_t)n * sizeof(double));
+        }
+    }
+
+    /* phase 5: rejuvenation (:640-652) */
+    if (a->use_rejuvenation && a->ksd_prev > a->rejuv_ksd_threshold && a->timestep > 10) {
+        double gsd = sqrt(gvar > 1e-6 ? gvar : 1e-6);
+        for (int i = 0; i < n; ++i) {
+            double u = v2_uniform(nz, nz->u_rej, i, tick, PURPOSE_REJUV, 1.0);
+            if (u < a->rejuv_prob) {
+                double z = v2_normal(nz, nz->z_rej, i, tick, PURPOSE_REJUV, 0.0);
+                h[i] = clamp_logvol((1.0 - a->rejuv_blend) * h[i] + a->rejuv_blend * (gmean + gsd * z));
+            }
+        }
+    }
+
+    /* phase 6: outputs (:656-707) */
+    double ks = 0.0, sh = 0.0, sv = 0.0, mx = -1e10, sw = 0.0;
+    for (int i = 0; i < n; ++i) {
+        ks += kp[i];
+        sh += h[i];
+        sv += exp(h[i] * 0.5);
+        if (log_w[i] > mx) mx = log_w[i];
+    }
+    for (int i = 0; i < n; ++i) sw += exp(log_w[i] - mx);
+    double k2 = ks / ((double)n * (double)n);
+    io->ksd = sqrt(k2 > 0.0 ? k2 : 0.0);
+    io->h_mean = sh / n;
+    io->vol = sv / n;
+    io->loglik = mx + log((sw > 1e-10 ? sw : 1e-10) / n);
+    if (a->use_adaptive_mu && a->timestep > 10)
+        io->mu_state = clamp_d(a->mu_ema_alpha * io->h_mean + (1.0 - a->mu_ema_alpha) * io->mu_state, -10.0, 0.0);
+    free(m);
+    free(s);
+    free(hs);
+    free(hn);
+    free(kp);
     return 0;
 }

@@ -148,6 +148,43 @@ int svpfo_run(const svpfo_cfg* c, svpfo_state* st, double* h, double* h_prev, do
               double* loglik, double* pred_loglik, double* vol, double* h_mean,
               double* ksd, double* bw, int* steps);
 
+/* ---- the single-block fused step (src/svpf_fully_fused_v2.cu:90-712) ----
+ * Scalar arguments of svpf_fully_fused_step_v2 (include/svpf_fully_fused_v2.cuh:43-116)
+ * in the reference's order, as doubles and ints. */
+typedef struct svpfo_v2_args {
+    int n, timestep;
+    int use_guide, use_guided_predict, use_guide_preserving;
+    int use_newton, use_full_newton;
+    int use_rejuvenation, use_adaptive_mu, use_adaptive_sigma;
+    int n_stein_steps, n_anneal_steps, stein_sign_mode;
+    int pad_;
+    double y_t, y_prev, h_mean_prev, vol_prev, ksd_prev;
+    double rho, sigma_z, mu, nu, lik_offset, student_t_const, student_t_implied_offset, gamma;
+    double delta_rho, delta_sigma, mim_jump_prob, mim_jump_scale;
+    double guide_strength_base, guide_strength_max, guide_innovation_threshold;
+    double guided_alpha_base, guided_alpha_shock, guided_innovation_thresh_predict;
+    double rejuv_ksd_threshold, rejuv_prob, rejuv_blend;
+    double mu_ema_alpha, sigma_boost_threshold, sigma_boost_max;
+    double step_size, temperature, rmsprop_rho, rmsprop_eps;
+} svpfo_v2_args;
+
+/* The call's device-pointer scalars: guide_mean / guide_var / mu_state are read
+ * and written, the rest are outputs. */
+typedef struct svpfo_v2_io {
+    double guide_mean, guide_var, mu_state;
+    double h_mean, vol, loglik, bandwidth, ksd;
+} svpfo_v2_io;
+
+/* One fused step.  h, h_prev, grad, log_w, v, inv_hess: length-n arrays (the
+ * reference's h, h_prev, grad_log_p, log_weights, d_grad_v, d_inv_hessian).
+ * Noise: with z_pred NULL every draw comes from the counter generator (predict
+ * normal and MIM uniform: purpose 0; Langevin normal of iteration k: purpose
+ * 16 + k; rejuvenation uniform and normal: purpose 1); otherwise z_pred[n],
+ * u_pred[n] (NULL: no jump), xi[xi_rows][n] (row k feeds iteration k; NULL or
+ * beyond xi_rows: 0), u_rej[n] (NULL: no rejuvenation), z_rej[n] (NULL: 0). */
+int svpfo_v2_step(const svpfo_v2_args* a, double* h, double* h_prev, double* grad, double* log_w, double* v,
+                  double* inv_hess, const svpfo_noise* nz, svpfo_v2_io* io);
+
 /* Counter-based generator: Philox4x32-10 (Salmon et al., SC'11). */
 void svpfo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* Student-t state noise from the counter generator (particle i, tick, filter). */

@@ -1,6 +1,6 @@
 """C ABI checks that need no GPU: the library loads, exports every function
-include/svpf.h declares, and the ctypes mirror of SVPFState matches the C
-layout field by field."""
+include/svpf.h and include/svpf_fully_fused_v2.h declare, and the ctypes
+mirrors of SVPFState and svpf_v2_noise match the C layouts field by field."""
 import ctypes
 import os
 import re
@@ -12,14 +12,16 @@ from svpf_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "svpf.h")
+HEADERS = [os.path.join(ROOT, "include", f) for f in ("svpf.h", "svpf_fully_fused_v2.h")]
 
 
 def declared_functions():
-    src = open(HEADER).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     names = set()
-    for m in re.finditer(r"^\s*(?!static)[A-Za-z_][\w\s\*]*?\b(svpf_\w+)\s*\(", src, flags=re.M):
-        names.add(m.group(1))
+    for hdr in HEADERS:
+        src = open(hdr).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^\s*(?!static)[A-Za-z_][\w\s\*]*?\b(svpf_\w+)\s*\(", src, flags=re.M):
+            names.add(m.group(1))
     return sorted(names)
 
 
@@ -119,3 +121,24 @@ def test_pysvpf_surface_matches_reference_module():
         assert hasattr(pysvpf.SVPF, k), k
     r = pysvpf.SVPFResult(1.0, 2.0, 0.0, 3.0, 4.0)
     assert (r.log_lik, r.vol_mean, r.h_mean, r.mu_estimate) == (1.0, 2.0, 3.0, 4.0)
+
+
+def test_v2_noise_layout_matches_header(tmp_path):
+    prog = tmp_path / "v2.c"
+    fields = [f for f, _ in _lib.SVPFV2Noise._fields_]
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "svpf_fully_fused_v2.h"', "int main(void) {",
+             'printf("size %zu\\n", sizeof(svpf_v2_noise));']
+    lines += [f'printf("{f} %zu\\n", offsetof(svpf_v2_noise, {f}));' for f in fields]
+    lines.append("return 0; }")
+    prog.write_text("\n".join(lines))
+    exe = tmp_path / "v2"
+    subprocess.run(["gcc", "-D__HIP_PLATFORM_AMD__", "-I", os.path.join(ROOT, "include"), "-I", "/opt/rocm/include",
+                    str(prog), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = dict(line.split() for line in out if line.strip())
+    assert int(got["size"]) == ctypes.sizeof(_lib.SVPFV2Noise)
+    for f in fields:
+        assert int(got[f]) == getattr(_lib.SVPFV2Noise, f).offset, f
+    L = _lib.load()
+    assert L.svpf_fully_fused_v2_supported(1024) and not L.svpf_fully_fused_v2_supported(1025)
+    assert not L.svpf_fully_fused_v2_supported(0)
