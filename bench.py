@@ -152,6 +152,40 @@ def time_single(torch, T=1000, warm=50):
     return us_device, float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
 
+def time_fused_v2(torch, n=512, T=1000, warm=100):
+    """The single-workgroup fused step (svpf_fully_fused_step_v2) with the
+    reference driver's arguments (10 Stein iterations over 5 stages, full
+    Newton, guide, MIM, rejuvenation; counter generator): back-to-back calls
+    (us/call from the wall clock around T calls), and the driver's loop, which
+    reads h_mean / vol / ksd back after every call (p50 / p99)."""
+    from svpf_amd.fused_v2 import FusedV2, v2_args
+
+    rng = np.random.default_rng(0)
+    y = (np.exp(-2.25) * rng.standard_normal(T + warm)).astype(np.float32)
+    f = FusedV2(np.full(n, -4.5, np.float32), mu_state=-4.5)
+    a = v2_args(n=n)
+    for t in range(warm):
+        f.step(dict(a, timestep=t, y_t=float(y[t])), None)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t in range(warm, warm + T):
+        f.step(dict(a, timestep=t, y_t=float(y[t])), None)
+    torch.cuda.synchronize()
+    us_call = (time.perf_counter() - t0) / T * 1e6
+    lat = []
+    prev = dict(y_prev=0.0, h_mean_prev=-4.5, vol_prev=float(np.exp(-2.25)), ksd_prev=0.1)
+    for t in range(warm, warm + 300):
+        t1 = time.perf_counter()
+        f.step(dict(a, timestep=t, y_t=float(y[t]), **prev), None)
+        o = f.outputs()
+        lat.append(time.perf_counter() - t1)
+        prev = dict(y_prev=float(y[t]), h_mean_prev=o["h_mean"], vol_prev=o["vol"], ksd_prev=o["ksd"])
+    lat = np.array(lat[20:]) * 1e6
+    return {"n": n, "us_per_call_back_to_back": us_call, "driver_loop_latency_us_p50": float(np.percentile(lat, 50)),
+            "driver_loop_latency_us_p99": float(np.percentile(lat, 99)),
+            "arguments": "test/test_fully_fused_v2.cu:91-113 (10 Stein iterations, 5 stages, full Newton)"}
+
+
 def time_large(torch, n, T=2000):
     """One full-feature filter (MIM, asymmetric rho, EKF guide, SVLD, KSD-adaptive
     beta, early stop, at most 8 Stein iterations) on the regime-shift series:
@@ -439,6 +473,8 @@ def profile_only(torch, what, B, K, W):
         print(json.dumps(calibration_demo(torch)), flush=True)
     elif what == "twofactor":
         log(json.dumps(time_twofactor(torch)))
+    elif what == "v2":
+        log(json.dumps(time_fused_v2(torch, T=K, warm=W)))
     elif what == "diff":
         theta, dys, dh0 = diff_workload()
         fwd, bwd, _ = time_diff(torch, theta, dys, dh0, 32)
@@ -455,12 +491,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--B", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
-    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "large"], default=None,
+    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "large", "v2"],
+                    default=None,
                     help="run one workload once and exit (for profiler runs): the N=2048 tile group of the "
                          "batched workload preceded by a 256 MiB device copy (traffic calibration), the "
                          "differentiable filter, the single N=512 filter, the two-factor filter, the "
-                         "two-factor calibration (prints its JSON), or the full-feature N=16384 filter "
-                         "over --steps ticks")
+                         "two-factor calibration (prints its JSON), the full-feature N=16384 filter "
+                         "over --steps ticks, or the fused single-workgroup step over --steps calls")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: filters are independent and this build runs on one MI355X; "
@@ -505,6 +542,8 @@ def main():
             "single_n512_tick_latency_us_p50": p50,
             "single_n512_tick_latency_us_p99": p99,
         }
+        log("fused single-workgroup step (svpf_fully_fused_step_v2), N=512")
+        extras["fused_v2"] = time_fused_v2(torch)
         log("differentiable filter B=256 N=1024 T=500")
         theta, dys, dh0 = diff_workload()
         T_d = dys.shape[1]

@@ -119,10 +119,9 @@ extern "C" hipError_t svpf_fully_fused_step_v2(
     A.sign_mode = stein_sign_mode;
     A.n = n;
     A.timestep = timestep;
-    // one workgroup of 256 lanes: column-split lanes for small n, several rows per lane above 256
-    if (n <= 64) return launch_v2<256, 1, 4>(A, stream);
-    if (n <= 128) return launch_v2<256, 1, 2>(A, stream);
-    if (n <= 256) return launch_v2<256, 1, 1>(A, stream);
-    if (n <= 512) return launch_v2<256, 2, 1>(A, stream);
-    return launch_v2<256, 4, 1>(A, stream);
+    // one workgroup of 1024 lanes (four waves per SIMD: the pair loops are latency-bound at
+    // one), with 2 or 4 lanes splitting each row's column loop below 1024 particles
+    if (n <= 256) return launch_v2<1024, 1, 4>(A, stream);
+    if (n <= 512) return launch_v2<1024, 1, 2>(A, stream);
+    return launch_v2<1024, 1, 1>(A, stream);
 }
