@@ -332,7 +332,10 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
  * checkpoint (x, v) every checkpoint_every ticks; svpf_diff_backward takes
  * dL/d pred_ll and writes dL/d theta (and dL/d h0), recomputing each
  * checkpoint segment into the tape.  Both return SVPF_OK or an SVPF_ERR_*
- * code.  svpf_diff_tape_floats is the tape size per filter. */
+ * code.  svpf_diff_tape_floats is the tape size per filter.  With the optional
+ * svpf_ddesc.drift ([T][S][n] floats per filter) the forward stores every
+ * iteration's Stein drift and the backward replays positions from it instead
+ * of recomputing the transport. */
 size_t svpf_diff_tape_floats(int n_particles, int n_stein, int checkpoint_every);
 int svpf_diff_forward(const void* config, const void* d_desc, int B, hipStream_t stream);
 int svpf_diff_backward(const void* config, const void* d_desc, int B, hipStream_t stream);

@@ -361,6 +361,8 @@ struct DiffEngine {
         const float ysq = y_t * y_t;
         const float isq = inv_s2 / sq;
         float* it = slot + 2 * (size_t)n;
+        // the recompute with a stored drift replays the positions: no records, no transport pass
+        const bool replay = RECORD && !SELECT && D.drift != nullptr;
         for (int k = 0; k < S; ++k, it += TP_ROWS * (size_t)n) {
             const float beta = S > 1 ? 0.3f + 0.7f * (float)k / (float)(S - 1) : 1.0f;
             const float eff = 0.1f * sqrtf(beta);
@@ -374,16 +376,18 @@ struct DiffEngine {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = row(r);
-                const float gp = -((s1[r] * isq) / (sr[r] + 1e-8f));
-                const float A = ysq / (safe_expf(h[r]) + 1e-8f) / nu;
-                const float opa = 1.0f + A;
-                const float gl = -0.5f + 0.5f * (nu + 1.0f) * A / opa - C.lik_offset;
-                const float g = fminf(fmaxf(gp + beta * gl, -10.0f), 10.0f);
-                const float cv = fminf(fmaxf(0.5f * (nu + 1.0f) * A / (opa * opa) + inv_s2, 0.1f), 100.0f);
-                const float s = 0.7f * g / cv;
                 xs[r] = h[r] * rs;
+                if (!replay) {
+                    const float gp = -((s1[r] * isq) / (sr[r] + 1e-8f));
+                    const float A = ysq / (safe_expf(h[r]) + 1e-8f) / nu;
+                    const float opa = 1.0f + A;
+                    const float gl = -0.5f + 0.5f * (nu + 1.0f) * A / opa - C.lik_offset;
+                    const float g = fminf(fmaxf(gp + beta * gl, -10.0f), 10.0f);
+                    const float cv = fminf(fmaxf(0.5f * (nu + 1.0f) * A / (opa * opa) + inv_s2, 0.1f), 100.0f);
+                    const float s = 0.7f * g / cv;
+                    if (i < n) rec[i] = make_float4(xs[r], s, (sgn2 / cv) * rs, 0.0f);
+                }
                 if (i < n) {
-                    rec[i] = make_float4(xs[r], s, (sgn2 / cv) * rs, 0.0f);
                     if (RECORD) {
                         const float Edd = s1[r] / sr[r];
                         const float m2 = s2[r] / sr[r];
@@ -396,13 +400,17 @@ struct DiffEngine {
                     }
                 }
             }
-            __syncthreads();  // records complete
             float acc[R];
-            transport(xs, acc);
+            if (!replay) {
+                __syncthreads();  // records complete
+                transport(xs, acc);
+            }
+            float* drift_k = D.drift ? D.drift + ((size_t)t * S + k) * n : nullptr;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = row(r);
-                const float p = acc[r] * inv_n;
+                const float p = replay ? (i < n ? drift_k[i] : 0.0f) : acc[r] * inv_n;
+                if (SELECT && drift_k && i < n) drift_k[i] = p;
                 const float vn = C.rms_rho * v[r] + (1.0f - C.rms_rho) * p * p;
                 const float drift = eff * p * rsqrtf(vn + C.rms_eps);
                 const float xi = (temp > 1e-6f && i < n) ? noise_xi(D, S, t, k, i) : 0.0f;

@@ -151,6 +151,8 @@ typedef struct svpf_ddesc {
     uint64_t seed;        // counter-generator key
     uint32_t filter_id;
     int32_t pad;
+    float* drift;         // [T][S][n] Stein drift phi of every iteration (forward output, optional): the
+                          // backward's recompute replays the positions from it instead of redoing the transport
 } svpf_ddesc;
 
 // ---------------------------------------------------------------------------

@@ -28,8 +28,8 @@ DCONFIG_DTYPE = np.dtype([("n", "<i4"), ("S", "<i4"), ("T", "<i4"), ("C", "<i4")
                           ("rms_rho", "<f4"), ("rms_eps", "<f4"), ("pad1", "<f4", (3,)), ("pad2", "<f4", (8,))])
 DDESC_DTYPE = np.dtype([(f, "<u8") for f in ("theta", "y", "h0", "z", "xi", "pred_ll", "ell2", "ckpt", "tape", "gpl",
                                               "gtheta", "gh0", "hT", "seed")]
-                       + [("filter_id", "<u4"), ("pad", "<i4")])
-assert DCONFIG_DTYPE.itemsize == 96 and DDESC_DTYPE.itemsize == 120
+                       + [("filter_id", "<u4"), ("pad", "<i4"), ("drift", "<u8")])
+assert DCONFIG_DTYPE.itemsize == 96 and DDESC_DTYPE.itemsize == 128
 
 
 def _bind(L):
@@ -76,6 +76,8 @@ class DiffBatch:
         self.gtheta = torch.zeros((B, 4), **f32)
         self.gh0 = torch.zeros((B, n), **f32)
         self.hT = torch.zeros((B, n), **f32)
+        # the Stein drift of every forward iteration: the backward replays positions from it
+        self.drift = torch.empty((B, T, self.S, n), **f32)
         self.noise = None
         if noise is not None:
             self.noise = {k: torch.as_tensor(noise[k], **f32).contiguous() for k in ("z", "xi")}
@@ -105,6 +107,7 @@ class DiffBatch:
             d["gtheta"] = ptr(self.gtheta, b, 4)
             d["gh0"] = ptr(self.gh0, b, n)
             d["hT"] = ptr(self.hT, b, n)
+            d["drift"] = ptr(self.drift, b, T * self.S * n)
             d["seed"] = seeds[b]
             d["filter_id"] = b
         self.desc = torch.from_numpy(desc.view(np.uint8).reshape(-1).copy()).to(dev)
