@@ -251,6 +251,8 @@ struct D2Engine {
         const float sgn2 = C.paper_sign ? 2.0f : -2.0f;
         const float temp = C.temperature;
         float* it = slot + 4 * (size_t)n;
+        // the recompute with a stored drift replays the positions: no records, no transport pass
+        const bool replay = RECORD && !SELECT && D.drift != nullptr;
         for (int k = 0; k < S; ++k, it += T2_ROWS * (size_t)n) {
             const float beta = S > 1 ? 0.3f + 0.7f * (float)k / (float)(S - 1) : 1.0f;
             const float eff = 0.1f * sqrtf(beta);
@@ -321,8 +323,10 @@ struct D2Engine {
                 uf[r] = hf[r] * rsf;
                 us[r] = hs[r] * rss;
                 if (i < n) {
-                    rec[i] = make_float4(uf[r], us[r], sf_, ss_);
-                    rec2[i] = make_float4(sgn2 * ca * rsf, sgn2 * cb * rss, sgn2 * cb * rsf, sgn2 * cd * rss);
+                    if (!replay) {
+                        rec[i] = make_float4(uf[r], us[r], sf_, ss_);
+                        rec2[i] = make_float4(sgn2 * ca * rsf, sgn2 * cb * rss, sgn2 * cb * rsf, sgn2 * cd * rss);
+                    }
                     if (RECORD) {
                         const float isr = 1.0f / sr[r];
                         const float Ef = s1f[r] * isr, Es = s1s[r] * isr;
@@ -344,12 +348,12 @@ struct D2Engine {
                     }
                 }
             }
-            __syncthreads();
             // transport
             float af[R], as[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) af[r] = as[r] = 0.f;
-            for (int j = 0; j < n; ++j) {
+            if (!replay) __syncthreads();
+            for (int j = 0; j < (replay ? 0 : n); ++j) {
                 const float4 e = rec[j];
                 const float4 q = rec2[j];
 #pragma unroll
@@ -361,10 +365,18 @@ struct D2Engine {
                     as[r] = fmaf(K, fmaf(K, Ps, e.w), as[r]);
                 }
             }
+            float* drift_k = D.drift ? D.drift + ((size_t)t * S + k) * 2 * n : nullptr;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = row(r);
-                const float pf = af[r] * inv_n, ps = as[r] * inv_n;
+                float pf = af[r] * inv_n, ps = as[r] * inv_n;
+                if (replay) {
+                    pf = i < n ? drift_k[i] : 0.0f;
+                    ps = i < n ? drift_k[n + i] : 0.0f;
+                } else if (SELECT && drift_k && i < n) {
+                    drift_k[i] = pf;
+                    drift_k[n + i] = ps;
+                }
                 const float vnf = C.rms_rho * vf[r] + (1.0f - C.rms_rho) * pf * pf;
                 const float vns = C.rms_rho * vs[r] + (1.0f - C.rms_rho) * ps * ps;
                 float xa = 0.f, xb = 0.f;

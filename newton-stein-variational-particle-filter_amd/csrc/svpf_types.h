@@ -195,6 +195,8 @@ typedef struct svpf_d2desc {
     uint64_t seed;
     uint32_t filter_id;
     int32_t pad;
+    float* drift;         // [T][S][2][n] Stein drift of every iteration (forward output, optional): the
+                          // backward's recompute replays the positions from it instead of redoing the transport
 } svpf_d2desc;
 
 #ifdef __cplusplus

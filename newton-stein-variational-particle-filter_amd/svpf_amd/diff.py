@@ -50,10 +50,14 @@ class DiffBatch:
     y [B, T] observations; h0 [B, n] initial particles; noise: None (counter
     generator keyed by ``seeds``) or a dict of pre-drawn tensors z [B, T, n]
     and xi [B, T, S, n].
+    drift_budget_gb: the forward keeps every iteration's Stein drift for the
+    backward (which then replays positions instead of recomputing the
+    transport) when that buffer fits this many GB.
     """
 
     def __init__(self, y, h0, n_stein=5, checkpoint_every=32, noise=None, seeds=None, gamma=0.0, lik_offset=0.0,
-                 temperature=1.0, rmsprop_rho=0.9, rmsprop_eps=1e-6, paper_sign=False, device="cuda"):
+                 temperature=1.0, rmsprop_rho=0.9, rmsprop_eps=1e-6, paper_sign=False, device="cuda",
+                 drift_budget_gb=32.0):
         self.L = _lib.load()
         _bind(self.L)
         dev = torch.device(device)
@@ -76,8 +80,10 @@ class DiffBatch:
         self.gtheta = torch.zeros((B, 4), **f32)
         self.gh0 = torch.zeros((B, n), **f32)
         self.hT = torch.zeros((B, n), **f32)
-        # the Stein drift of every forward iteration: the backward replays positions from it
-        self.drift = torch.empty((B, T, self.S, n), **f32)
+        # the Stein drift of every forward iteration (the backward replays positions from it),
+        # kept when it fits the budget; otherwise the backward recomputes the transport
+        drift_gb = B * T * self.S * n * 4 / 1e9
+        self.drift = torch.empty((B, T, self.S, n), **f32) if drift_gb <= drift_budget_gb else None
         self.noise = None
         if noise is not None:
             self.noise = {k: torch.as_tensor(noise[k], **f32).contiguous() for k in ("z", "xi")}
@@ -107,7 +113,8 @@ class DiffBatch:
             d["gtheta"] = ptr(self.gtheta, b, 4)
             d["gh0"] = ptr(self.gh0, b, n)
             d["hT"] = ptr(self.hT, b, n)
-            d["drift"] = ptr(self.drift, b, T * self.S * n)
+            if self.drift is not None:
+                d["drift"] = ptr(self.drift, b, T * self.S * n)
             d["seed"] = seeds[b]
             d["filter_id"] = b
         self.desc = torch.from_numpy(desc.view(np.uint8).reshape(-1).copy()).to(dev)

@@ -28,8 +28,8 @@ D2CONFIG_DTYPE = np.dtype([("n", "<i4"), ("S", "<i4"), ("T", "<i4"), ("C", "<i4"
                            ("rms_eps", "<f4"), ("pad1", "<f4", (8,))])
 D2DESC_DTYPE = np.dtype([(f, "<u8") for f in ("theta", "y", "x0", "z", "xi", "out", "ell2", "ckpt", "tape", "gpl",
                                               "gtheta", "gx0", "xT", "seed")]
-                        + [("filter_id", "<u4"), ("pad", "<i4")])
-assert D2CONFIG_DTYPE.itemsize == 80 and D2DESC_DTYPE.itemsize == 120
+                        + [("filter_id", "<u4"), ("pad", "<i4"), ("drift", "<u8")])
+assert D2CONFIG_DTYPE.itemsize == 80 and D2DESC_DTYPE.itemsize == 128
 
 OUTW = 8
 O_PRED_LL, O_HMEAN, O_FMEAN, O_SMEAN, O_VOL, O_BWF, O_BWS = range(7)
@@ -53,10 +53,14 @@ class TwoFactorBatch:
 
     y [B, T]; x0 [B, 2, n] initial factors; noise None (counter generator keyed
     by ``seeds``) or pre-drawn z [B, T, 2, n] and xi [B, T, S, 2, n].
+    drift_budget_gb: the forward keeps every iteration's Stein drift for the
+    backward (which then replays positions instead of recomputing the
+    transport) when that buffer fits this many GB.
     """
 
     def __init__(self, y, x0, n_stein=5, checkpoint_every=32, noise=None, seeds=None, lik_offset=0.0,
-                 temperature=1.0, rmsprop_rho=0.9, rmsprop_eps=1e-6, paper_sign=False, device="cuda"):
+                 temperature=1.0, rmsprop_rho=0.9, rmsprop_eps=1e-6, paper_sign=False, device="cuda",
+                 drift_budget_gb=32.0):
         self.L = _lib.load()
         _bind(self.L)
         dev = torch.device(device)
@@ -79,6 +83,10 @@ class TwoFactorBatch:
         self.gtheta = torch.zeros((B, 6), **f32)
         self.gx0 = torch.zeros((B, 2, n), **f32)
         self.xT = torch.zeros((B, 2, n), **f32)
+        # the Stein drift of every forward iteration (the backward replays positions from it),
+        # kept when it fits the budget; otherwise the backward recomputes the transport
+        drift_gb = B * T * self.S * 2 * n * 4 / 1e9
+        self.drift = torch.empty((B, T, self.S, 2, n), **f32) if drift_gb <= drift_budget_gb else None
         self.noise = None
         if noise is not None:
             self.noise = {k: torch.as_tensor(noise[k], **f32).contiguous() for k in ("z", "xi")}
@@ -108,6 +116,8 @@ class TwoFactorBatch:
             d["gtheta"] = ptr(self.gtheta, b, 6)
             d["gx0"] = ptr(self.gx0, b, 2 * n)
             d["xT"] = ptr(self.xT, b, 2 * n)
+            if self.drift is not None:
+                d["drift"] = ptr(self.drift, b, T * self.S * 2 * n)
             d["seed"] = seeds[b]
             d["filter_id"] = b
         self.desc = torch.from_numpy(desc.view(np.uint8).reshape(-1).copy()).to(dev)

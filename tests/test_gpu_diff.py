@@ -35,9 +35,10 @@ def inputs(n, T, S, thetas, seed=0):
     return np.stack(ys), np.stack(h0s), np.stack(zs), np.stack(xis)
 
 
-def make_batch(n, T, S, thetas, C, gamma=0.0, sign=False, seed=0):
+def make_batch(n, T, S, thetas, C, gamma=0.0, sign=False, seed=0, drift_budget_gb=32.0):
     y, h0, z, xi = inputs(n, T, S, thetas, seed)
-    bt = DiffBatch(y, h0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi}, gamma=gamma, paper_sign=sign)
+    bt = DiffBatch(y, h0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi}, gamma=gamma, paper_sign=sign,
+                   drift_budget_gb=drift_budget_gb)
     return bt, (y, h0, z, xi)
 
 
@@ -118,10 +119,14 @@ def test_adjoint_matches_oracle_central_differences(torch_cuda):
 
 
 def test_gradient_independent_of_checkpoint_interval(torch_cuda):
+    """Bit-identical gradients for every checkpoint interval, with the backward
+    replaying positions from the stored Stein drift or (no drift buffer)
+    recomputing the transport."""
     n, T, S = 128, 9, 5
     grads = []
-    for C in (1, 2, 4, 9):
-        bt, _ = make_batch(n, T, S, GRAD_THETAS, C, seed=7)
+    for C, budget in ((1, 32.0), (2, 32.0), (4, 32.0), (9, 32.0), (2, 0.0), (4, 0.0)):
+        bt, _ = make_batch(n, T, S, GRAD_THETAS, C, seed=7, drift_budget_gb=budget)
+        assert (bt.drift is None) == (budget == 0.0)
         theta = torch.tensor(GRAD_THETAS, device="cuda", requires_grad=True)
         (-predictive_loglik(theta, bt).sum()).backward()
         grads.append(theta.grad.cpu().numpy())

@@ -37,9 +37,10 @@ def inputs(n, T, S, thetas, seed=0):
     return np.stack(ys), np.stack(x0s), np.stack(zs), np.stack(xis)
 
 
-def make(n, T, S, thetas, C, seed=0, sign=False):
+def make(n, T, S, thetas, C, seed=0, sign=False, drift_budget_gb=32.0):
     y, x0, z, xi = inputs(n, T, S, thetas, seed)
-    bt = TwoFactorBatch(y, x0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi}, paper_sign=sign)
+    bt = TwoFactorBatch(y, x0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi}, paper_sign=sign,
+                        drift_budget_gb=drift_budget_gb)
     return bt, (y, x0, z, xi)
 
 
@@ -87,10 +88,13 @@ def test_adjoint_matches_torch_autograd(torch_cuda, sign):
 
 
 def test_gradient_independent_of_checkpoint_interval(torch_cuda):
+    """Bit-identical gradients for every checkpoint interval, replaying the
+    stored Stein drift or (no drift buffer) recomputing the transport."""
     n, T, S = 128, 7, 5
     grads = []
-    for C in (1, 3, 7):
-        bt, _ = make(n, T, S, GRAD_THETAS, C, seed=7)
+    for C, budget in ((1, 32.0), (3, 32.0), (7, 32.0), (3, 0.0)):
+        bt, _ = make(n, T, S, GRAD_THETAS, C, seed=7, drift_budget_gb=budget)
+        assert (bt.drift is None) == (budget == 0.0)
         theta = torch.tensor(GRAD_THETAS, device="cuda", requires_grad=True)
         (-predictive_loglik_2d(theta, bt).sum()).backward()
         grads.append(theta.grad.cpu().numpy())
