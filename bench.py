@@ -161,7 +161,7 @@ def time_fused_v2(torch, n=512, T=1000, warm=100):
     from svpf_amd.fused_v2 import FusedV2, v2_args
 
     rng = np.random.default_rng(0)
-    y = (np.exp(-2.25) * rng.standard_normal(T + warm)).astype(np.float32)
+    y = (np.exp(-2.25) * rng.standard_normal(warm + max(T, 300))).astype(np.float32)
     f = FusedV2(np.full(n, -4.5, np.float32), mu_state=-4.5)
     a = v2_args(n=n)
     for t in range(warm):
@@ -184,6 +184,24 @@ def time_fused_v2(torch, n=512, T=1000, warm=100):
     return {"n": n, "us_per_call_back_to_back": us_call, "driver_loop_latency_us_p50": float(np.percentile(lat, 50)),
             "driver_loop_latency_us_p99": float(np.percentile(lat, 99)),
             "arguments": "test/test_fully_fused_v2.cu:91-113 (10 Stein iterations, 5 stages, full Newton)"}
+
+
+def time_extended(torch, B=512, n=512, warm=2, T=10):
+    """The batched kernel's extended instantiation: B filters of n particles
+    with the plain configuration, with the Heun step, with the Student-t state
+    (nu_s = 5) and with both, each as one batched run (filter-steps/s)."""
+    from svpf_amd import config, data
+
+    y = np.stack([data.sv_series(warm + T, seed=b)[0] for b in range(B)])
+    h0s = [data.initial_particles(n, -1.0, 0.97, 0.15, seed=100 + b) for b in range(B)]
+    out = {"workload": f"B={B} filters, N={n}, plain configuration (5 Stein iterations, local Newton, SVLD)"}
+    for name, feats in (("plain", {}), ("heun", dict(use_heun=1)),
+                        ("student_t_state", dict(use_student_t_state=1, nu_state=5.0)),
+                        ("heun_student_t_state", dict(use_heun=1, use_student_t_state=1, nu_state=5.0))):
+        cfgs = [config.plain_config(n=n, **feats) for _ in range(B)]
+        wall, _, _ = time_batched(torch, cfgs, y, h0s, warm, T)
+        out[name + "_filter_steps_per_s"] = B * T / wall
+    return out
 
 
 def time_large(torch, n, T=2000):
@@ -473,6 +491,8 @@ def profile_only(torch, what, B, K, W):
         print(json.dumps(calibration_demo(torch)), flush=True)
     elif what == "twofactor":
         log(json.dumps(time_twofactor(torch)))
+    elif what == "extended":
+        log(json.dumps(time_extended(torch, T=K)))
     elif what == "v2":
         log(json.dumps(time_fused_v2(torch, T=K, warm=W)))
     elif what == "diff":
@@ -491,13 +511,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--B", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
-    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "large", "v2"],
+    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "large", "v2",
+                                       "extended"],
                     default=None,
                     help="run one workload once and exit (for profiler runs): the N=2048 tile group of the "
                          "batched workload preceded by a 256 MiB device copy (traffic calibration), the "
                          "differentiable filter, the single N=512 filter, the two-factor filter, the "
                          "two-factor calibration (prints its JSON), the full-feature N=16384 filter "
-                         "over --steps ticks, or the fused single-workgroup step over --steps calls")
+                         "over --steps ticks, the fused single-workgroup step over --steps calls, or the "
+                         "Heun / Student-t batched runs over --steps ticks")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: filters are independent and this build runs on one MI355X; "
@@ -542,6 +564,8 @@ def main():
             "single_n512_tick_latency_us_p50": p50,
             "single_n512_tick_latency_us_p99": p99,
         }
+        log("extended features (Heun step, Student-t state), B=512 N=512")
+        extras["extended_features"] = time_extended(torch)
         log("fused single-workgroup step (svpf_fully_fused_step_v2), N=512")
         extras["fused_v2"] = time_fused_v2(torch)
         log("differentiable filter B=256 N=1024 T=500")
