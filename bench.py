@@ -640,7 +640,7 @@ def main():
             "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE of the 5-tick launch, "
                               "profiles/svpf_batched_kernel_n2048_group_trace.md",
             "kernel": f"svpf_batched_kernel<{int(groups['bs'][0])},{TILES[int(groups['tile'][0])][0]},"
-                      f"{TILES[int(groups['tile'][0])][1]}> (N=2048 group, {nf_dom} filters, grid {int(groups['grid'][0])})",
+                      f"{TILES[int(groups['tile'][0])][1]},false> (N=2048 group, {nf_dom} filters, grid {int(groups['grid'][0])})",
             "kernel_ms": ms_dom,
             "flops_per_launch": flops_dom,
         },
