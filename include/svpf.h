@@ -267,6 +267,17 @@ int svpf_get_last_outputs(const SVPFState* state, float out8[8]);
  * on one XCD, 1 spreads them.  Pass -1 for the defaults. */
 int svpf_set_coop_shape(SVPFState* state, int lanes_per_row, int xcd_stride);
 int svpf_get_coop_shape(const SVPFState* state, int* lanes_per_row, int* workgroups, int* xcd_stride);
+/* No reference counterpart: the cooperative kernel runs as a plain launch
+ * (its workgroups are co-resident on an otherwise free GPU); when an exchange
+ * times out -- nothing is written back then -- the step is repeated with a
+ * cooperative launch (co-residency guaranteed), which the filter keeps.
+ * svpf_coop_launch_is_cooperative reports 1 once that happened.
+ * svpf_set_coop_spin_limit bounds each exchange wait at `polls` polls (0: the
+ * default 2^20, at most 2^24); a negative value makes every exchange wait for
+ * a tag nobody publishes and time out after -polls polls (failure-path
+ * testing, at least -2^20). */
+int svpf_coop_launch_is_cooperative(const SVPFState* state);
+int svpf_set_coop_spin_limit(SVPFState* state, int polls);
 
 /* No reference counterpart: the kernel path serving one filter.
  *   SVPF_PATH_SINGLE  one workgroup holds the filter in LDS;

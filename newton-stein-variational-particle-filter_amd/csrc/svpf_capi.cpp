@@ -77,6 +77,14 @@ struct svpf_impl {
     float med_r2 = 0.0f;  // last median pairwise r^2 (a search seed, not a result)
     bool captured_coop = false;
     bool coop_graph_failed = false;
+    // The cooperative kernel runs as a plain launch: its workgroups are co-resident on an
+    // otherwise free GPU, and a cooperative launch costs about 20 us more per tick (100 us more
+    // once other streams exist).  An exchange that times out (workgroups not all resident)
+    // writes nothing back; the tick is then repeated with a cooperative launch, which this
+    // filter keeps from then on.
+    bool coop_cooperative = false;
+    bool captured_cooperative = false;
+    int coop_spin_limit = 0;
     // grid path (workspace allocated for 2 <= n <= SVPF_GRID_MAX_PARTICLES)
     int grid_mode = SVPF_GRID_AUTO;
     float* dgrid_ws = nullptr;
@@ -209,6 +217,7 @@ void pack_params(const SVPFState* s, const SVPFParams* p, svpf_fparams* o) {
     if (s->use_heun) f |= SVPF_F_HEUN;
     if (s->use_student_t_state) f |= SVPF_F_STUDENT_STATE;
     o->nu_state = s->nu_state;
+    o->coop_spin_limit = s->impl->coop_spin_limit;
     o->n = s->n_particles;
     o->flags = f;
     o->n_anneal = s->n_anneal_steps;
@@ -370,7 +379,7 @@ hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
                                 I->n, s->stein_max_steps, t0, t1, cs);
     if (use_coop(s))
         return svpf_launch_coop(I->coop_cs, I->coop_G, I->coop_stride, I->dp(), I->ds(), I->dd(), I->d_h(),
-                                I->d_hp(), I->d_v(), I->xbuf, I->n, t0, t1, 1, cs);
+                                I->d_hp(), I->d_v(), I->xbuf, I->n, t0, t1, I->coop_cooperative ? 1 : 0, cs);
     const int tile = I->tile | ((s->use_heun || s->use_student_t_state) ? SVPF_TILE_EXT : 0);
     return svpf_launch_batched(I->bs, tile, 1, I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->dorder,
                                1, nullptr, I->n, t0, t1, cs);
@@ -469,7 +478,8 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     const bool coop = use_coop(s);
     const int path = step_path(s);
     // the captured launch sequence depends on the path and, on the grid path, on stein_max_steps
-    if (I->captured && (I->captured_path != path || (path == SVPF_PATH_GRID && I->captured_smax != s->stein_max_steps)))
+    if (I->captured && (I->captured_path != path || (path == SVPF_PATH_GRID && I->captured_smax != s->stein_max_steps) ||
+                        (path == SVPF_PATH_COOP && I->captured_cooperative != I->coop_cooperative)))
         svpf_graph_invalidate(s);
     if (use_graph && coop && I->coop_graph_failed) use_graph = false;
     if (use_graph) {
@@ -489,6 +499,7 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
                     I->captured_coop = coop;
                     I->captured_path = path;
                     I->captured_smax = s->stein_max_steps;
+                    I->captured_cooperative = I->coop_cooperative;
                 } else if (g) {
                     hipGraphDestroy(g);
                 }
@@ -505,6 +516,16 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     }
     if (e == hipSuccess) e = use_graph ? hipGraphLaunch(I->exec, cs) : enqueue_tick(s, cs);
     if (e == hipSuccess) e = hipStreamSynchronize(cs);
+    if (e == hipSuccess && I->hs()->status == 2 && coop && !I->coop_cooperative) {
+        // a plain launch's exchange timed out; repeat the tick with a cooperative launch, on
+        // fresh exchange epochs
+        I->coop_cooperative = true;
+        I->epoch += 1u << 16;
+        pack_fstate(s, y_prev, I->hs());
+        svpf_graph_invalidate(s);
+        e = enqueue_tick(s, cs);
+        if (e == hipSuccess) e = hipStreamSynchronize(cs);
+    }
     if (e != hipSuccess) {
         I->last_hip = e;
         set_err(s, SVPF_ERR_HIP);
@@ -537,12 +558,23 @@ int run_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, flo
     pack_fstate(s, y_prev0, I->hs());
     fill_desc(s, I->hd(), d_obs, I->dseq_out, s->timestep);
     hipStream_t cs = s->stream;
-    hipError_t e = hipMemcpyAsync(I->dblock, I->hblock, kUploadBytes, hipMemcpyHostToDevice, cs);
-    if (e == hipSuccess) e = launch_step_kernel(s, 0, T, cs);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(I->hblock + kBlockFstate, I->dblock + kBlockFstate, sizeof(svpf_fstate),
-                           hipMemcpyDeviceToHost, cs);
-    if (e == hipSuccess) e = hipStreamSynchronize(cs);
+    const bool coop = use_coop(s);
+    hipError_t e = hipSuccess;
+    for (int attempt = 0; attempt < 2 && e == hipSuccess; ++attempt) {
+        e = hipMemcpyAsync(I->dblock, I->hblock, kUploadBytes, hipMemcpyHostToDevice, cs);
+        if (e == hipSuccess) e = launch_step_kernel(s, 0, T, cs);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(I->hblock + kBlockFstate, I->dblock + kBlockFstate, sizeof(svpf_fstate),
+                               hipMemcpyDeviceToHost, cs);
+        if (e == hipSuccess) e = hipStreamSynchronize(cs);
+        // a plain launch's exchange timed out (nothing was written back): repeat the run with a
+        // cooperative launch, on fresh exchange epochs
+        if (e != hipSuccess || I->hs()->status != 2 || !coop || I->coop_cooperative) break;
+        I->coop_cooperative = true;
+        I->epoch += 1u << 16;
+        pack_fstate(s, y_prev0, I->hs());
+        svpf_graph_invalidate(s);
+    }
     if (e != hipSuccess) {
         I->last_hip = e;
         return SVPF_ERR_HIP;
@@ -1053,6 +1085,18 @@ int svpf_set_coop_shape(SVPFState* s, int lanes_per_row, int xcd_stride) {
     I->coop_stride = trial.coop_stride;
     I->coop_ok = true;
     svpf_graph_invalidate(s);
+    return set_err(s, SVPF_OK);
+}
+
+int svpf_coop_launch_is_cooperative(const SVPFState* s) {
+    if (!s || !s->impl) return -1;
+    return s->impl->coop_cooperative ? 1 : 0;
+}
+
+int svpf_set_coop_spin_limit(SVPFState* s, int polls) {
+    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    if (polls < -(1 << 20) || polls > (1 << 24)) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+    s->impl->coop_spin_limit = polls;
     return set_err(s, SVPF_OK);
 }
 

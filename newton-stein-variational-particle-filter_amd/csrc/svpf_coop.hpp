@@ -40,7 +40,9 @@ SVPF_DEV uint64_t tag_word(float v, uint32_t epoch) { return ((uint64_t)epoch <<
 template <int BS, int CS>
 struct CoopEngine {
     static constexpr int ROWS = BS / CS;
-    static constexpr long kSpinLimit = 1L << 20;
+    static constexpr long kSpinLimit = 1L << 20;  // default bound of an exchange wait (polls)
+    long spin_limit = kSpinLimit;
+    uint32_t epoch_xor = 0;  // nonzero: the gathers wait for a tag nobody publishes (forced time-out)
 
     // LDS
     float4* hsc;     // [n]  transport records of all particles
@@ -94,14 +96,14 @@ struct CoopEngine {
             for (int u = 0; u < 4; ++u) {
                 int k = k0 + u * BS;
                 w[u] = k < cnt ? __hip_atomic_load(buf + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : ((uint64_t)epoch << 32);
+                               : ((uint64_t)(epoch ^ epoch_xor) << 32);
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 int k = k0 + u * BS;
                 long spins = 0;
-                while ((uint32_t)(w[u] >> 32) != epoch) {
-                    if (++spins > kSpinLimit) {
+                while ((uint32_t)(w[u] >> 32) != (epoch ^ epoch_xor)) {
+                    if (++spins > spin_limit) {
                         ok = false;
                         break;
                     }
@@ -142,6 +144,10 @@ struct CoopEngine {
         const uint32_t F = P.flags;
         const bool rng_mode = (D.z_pred == nullptr);
         CounterRng rng{(uint32_t)D.seed, (uint32_t)(D.seed >> 32), D.filter_id};
+        // coop_spin_limit > 0: bound of each exchange wait in polls; < 0: a forced time-out after
+        // -coop_spin_limit polls (failure-path testing); 0: the default bound
+        spin_limit = P.coop_spin_limit != 0 ? (long)abs(P.coop_spin_limit) : kSpinLimit;
+        epoch_xor = P.coop_spin_limit < 0 ? 0x80000000u : 0u;
         __shared__ svpf_fstate fs;  // this workgroup's copy of the scalar state
         if (threadIdx.x == 0) {
             fs = *io.S;

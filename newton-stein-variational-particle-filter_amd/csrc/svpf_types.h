@@ -63,7 +63,9 @@ typedef struct svpf_fparams {
     float sb_thr, sb_max, ksd_thr, st_const;
     float implied_offset, base_step;
     float nu_state;
-    float pad1[21];
+    int32_t coop_spin_limit;  // polls before a cooperative-kernel exchange times out (0: 2^20;
+                              // negative: a forced time-out after that many polls)
+    float pad1[20];
 } svpf_fparams;
 
 // Per-filter scalar state carried between ticks (host fields of SVPFState and

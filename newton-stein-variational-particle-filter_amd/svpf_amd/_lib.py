@@ -97,6 +97,7 @@ EXPORTED = (
     "svpf_last_error_string", "svpf_diff_tape_floats", "svpf_diff_forward", "svpf_diff_backward",
     "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
     "svpf_set_grid_path", "svpf_get_path", "svpf_fully_fused_step_v2", "svpf_fully_fused_v2_supported",
+    "svpf_coop_launch_is_cooperative", "svpf_set_coop_spin_limit",
 )
 
 # svpf_fully_fused_step_v2 argument kinds, in order (include/svpf_fully_fused_v2.h)
@@ -193,6 +194,10 @@ def load(path: str | None = None):
     L.svpf_set_grid_path.argtypes = [SP, c_int]
     L.svpf_get_path.restype = c_int
     L.svpf_get_path.argtypes = [SP]
+    L.svpf_coop_launch_is_cooperative.restype = c_int
+    L.svpf_coop_launch_is_cooperative.argtypes = [SP]
+    L.svpf_set_coop_spin_limit.restype = c_int
+    L.svpf_set_coop_spin_limit.argtypes = [SP, c_int]
     L.svpf_get_coop_shape.argtypes = [SP, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
     kinds = {"p": c_void_p, "f": c_float, "i": c_int, "noise": ctypes.POINTER(SVPFV2Noise)}
     L.svpf_fully_fused_step_v2.restype = c_int

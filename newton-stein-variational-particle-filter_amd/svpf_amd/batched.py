@@ -30,7 +30,7 @@ _PARAM_FLOATS = ("mu", "rho", "sigma_z", "gamma", "nu", "lik_offset", "temperatu
                  "mu_max", "sb_thr", "sb_max", "ksd_thr", "st_const", "implied_offset", "base_step")
 FPARAMS_DTYPE = np.dtype([("n", "<i4"), ("flags", "<u4"), ("n_anneal", "<i4"), ("stein_min", "<i4"),
                           ("stein_max", "<i4"), ("bandwidth_mode", "<i4"), ("pad0", "<i4", (2,))]
-                         + [(f, "<f4") for f in _PARAM_FLOATS] + [("nu_state", "<f4"), ("pad1", "<f4", (21,))])
+                         + [(f, "<f4") for f in _PARAM_FLOATS] + [("nu_state", "<f4"), ("coop_spin_limit", "<i4"), ("pad1", "<f4", (20,))])
 FSTATE_DTYPE = np.dtype([("timestep", "<i4"), ("guide_initialized", "<i4"), ("stein_steps_used", "<i4"),
                          ("status", "<i4")]
                         + [(f, "<f4") for f in ("ksd_prev", "vol_prev", "h_mean_prev", "guide_mean", "guide_var",

@@ -249,3 +249,45 @@ def test_coop_device_run_equals_per_tick(torch_cuda):
     f2.run_device(dev(torch, y), ll)
     assert np.array_equal(ll.cpu().numpy(), rec[:, 0].astype(np.float32))
     assert np.array_equal(f1.particles(), f2.particles())
+
+
+def test_coop_exchange_timeout_repeats_with_a_cooperative_launch(torch_cuda):
+    """The cooperative kernel runs as a plain launch.  A forced time-out (the
+    exchanges wait for a tag nobody publishes, 4 polls) writes nothing back; the
+    step is repeated with a cooperative launch (forced to time out as well, so it
+    reports SVPF_ERR_TIMEOUT) and the filter keeps cooperative launches.  With
+    the default bound its next tick equals that of a filter that never timed out."""
+    from svpf_amd import _lib
+
+    n = 512
+    cfg = config.plain_config(n=n)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=31)
+    noise = data.predraw_noise(1, n, 5, seed=32)
+    y0 = 0.37
+
+    def make():
+        f = Filter(cfg, path="coop")
+        f.set_particles(h0)
+        f.set_noise(dev(torch_cuda, noise["z_pred"]), dev(torch_cuda, noise["xi"]))
+        assert f.path() == "coop"
+        return f
+
+    f = make()
+    L = f.L
+    assert L.svpf_coop_launch_is_cooperative(f.sp) == 0
+    assert L.svpf_set_coop_spin_limit(f.sp, -4) == _lib.SVPF_OK
+    print("coop exchange time-out", flush=True)
+    with pytest.raises(RuntimeError, match="time"):
+        f.step(y0, 0.0)
+    assert L.svpf_last_error(f.sp) == _lib.SVPF_ERR_TIMEOUT
+    assert L.svpf_coop_launch_is_cooperative(f.sp) == 1
+    assert np.array_equal(f.particles(), h0.astype(np.float32))
+    assert f.state.timestep == 0
+    assert L.svpf_set_coop_spin_limit(f.sp, 0) == _lib.SVPF_OK
+    f.step(y0, 0.0)
+    g = make()
+    g.step(y0, 0.0)
+    assert L.svpf_coop_launch_is_cooperative(g.sp) == 0
+    assert np.array_equal(f.particles(), g.particles())
+    assert np.array_equal(f.last_outputs(), g.last_outputs())
+    assert L.svpf_set_coop_spin_limit(f.sp, -(1 << 21)) == _lib.SVPF_ERR_INVALID_ARGUMENT
