@@ -89,6 +89,7 @@ struct svpf_impl {
     int grid_mode = SVPF_GRID_AUTO;
     float* dgrid_ws = nullptr;
     int captured_path = -1, captured_smax = -1;
+    unsigned captured_ext = 0;
 
     svpf_fparams* hp() { return reinterpret_cast<svpf_fparams*>(hblock + kBlockParams); }
     svpf_fstate* hs() { return reinterpret_cast<svpf_fstate*>(hblock + kBlockFstate); }
@@ -329,9 +330,6 @@ void fill_desc(SVPFState* s, svpf_fdesc* d, const float* y, float* out, int tick
 
 int check_supported(SVPFState* s) {
     if (s->use_two_factor) return SVPF_ERR_UNSUPPORTED;
-    // Heun and the Student-t state run in the single-workgroup engine only
-    if ((s->use_heun || s->use_student_t_state) && (s->impl->tile < 0 || s->impl->grid_mode == SVPF_GRID_ON))
-        return SVPF_ERR_UNSUPPORTED;
     if (s->n_particles != s->impl->n) return SVPF_ERR_INVALID_ARGUMENT;
     if (s->bandwidth_mode < 0 || s->bandwidth_mode > 2) return SVPF_ERR_INVALID_ARGUMENT;
     if (s->n_anneal_steps < 1 && s->use_annealing) return SVPF_ERR_INVALID_ARGUMENT;
@@ -359,7 +357,6 @@ constexpr int kCoopMinN = 256;
 bool use_grid(const SVPFState* s) {
     const svpf_impl* I = s->impl;
     if (!I->dgrid_ws) return false;
-    if (s->use_heun || s->use_student_t_state) return I->tile < 0;  // single-workgroup engine features
     if (I->tile < 0 || I->grid_mode == SVPF_GRID_ON) return true;
     if (I->grid_mode == SVPF_GRID_OFF) return false;
     if (I->n >= kGridAutoMinN) return true;
@@ -367,6 +364,11 @@ bool use_grid(const SVPFState* s) {
 }
 
 bool use_coop(const SVPFState* s) { return !use_grid(s) && coop_applies(s); }
+
+// launch-sequence variants of the grid path
+unsigned grid_ext(const SVPFState* s) {
+    return (s->use_heun ? SVPF_GRID_EXT_HEUN : 0u) | (s->use_student_t_state ? SVPF_GRID_EXT_STUDENT : 0u);
+}
 
 int step_path(const SVPFState* s) {
     return use_grid(s) ? SVPF_PATH_GRID : (use_coop(s) ? SVPF_PATH_COOP : SVPF_PATH_SINGLE);
@@ -376,7 +378,7 @@ hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
     svpf_impl* I = s->impl;
     if (use_grid(s))
         return svpf_launch_grid(I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->d_lw(), I->dgrid_ws,
-                                I->n, s->stein_max_steps, t0, t1, cs);
+                                I->n, s->stein_max_steps, grid_ext(s), t0, t1, cs);
     if (use_coop(s))
         return svpf_launch_coop(I->coop_cs, I->coop_G, I->coop_stride, I->dp(), I->ds(), I->dd(), I->d_h(),
                                 I->d_hp(), I->d_v(), I->xbuf, I->n, t0, t1, I->coop_cooperative ? 1 : 0, cs);
@@ -478,7 +480,9 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     const bool coop = use_coop(s);
     const int path = step_path(s);
     // the captured launch sequence depends on the path and, on the grid path, on stein_max_steps
-    if (I->captured && (I->captured_path != path || (path == SVPF_PATH_GRID && I->captured_smax != s->stein_max_steps) ||
+    if (I->captured && (I->captured_path != path ||
+                        (path == SVPF_PATH_GRID && (I->captured_smax != s->stein_max_steps ||
+                                                    I->captured_ext != grid_ext(s))) ||
                         (path == SVPF_PATH_COOP && I->captured_cooperative != I->coop_cooperative)))
         svpf_graph_invalidate(s);
     if (use_graph && coop && I->coop_graph_failed) use_graph = false;
@@ -500,6 +504,7 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
                     I->captured_path = path;
                     I->captured_smax = s->stein_max_steps;
                     I->captured_cooperative = I->coop_cooperative;
+                    I->captured_ext = grid_ext(s);
                 } else if (g) {
                     hipGraphDestroy(g);
                 }

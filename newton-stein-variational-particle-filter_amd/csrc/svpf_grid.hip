@@ -278,7 +278,9 @@ __global__ void grid_prologue(const svpf_fparams* __restrict__ Pp, svpf_fstate* 
     g->guide_mean = S.guide_mean;
     g->guide_std = sqrtf(fmaxf(S.guide_var, 1e-6f));
     g->inv_s2 = 1.0f / (sigma * sigma);
-    g->sq = sqrtf(1.4426950408889634f * 0.5f * g->inv_s2);
+    // the means are scaled by sqrt(log2(e) / 2 sigma^2) for the Gaussian state prior, by
+    // 1 / sqrt(nu_s sigma^2) for the Student-t one
+    g->sq = (F & SVPF_F_STUDENT_STATE) ? sqrtf(g->inv_s2 / P.nu_state) : sqrtf(1.4426950408889634f * 0.5f * g->inv_s2);
     g->budget = budget;
     g->stop = 0;
     g->total = 0;
@@ -337,6 +339,16 @@ __global__ __launch_bounds__(kGB) void grid_predict(const svpf_fparams* __restri
             rng.draw((uint32_t)i, (uint32_t)S.timestep, SVPF_PURPOSE_PREDICT, o);
             z = normal_from_bits(o[0], o[1]);
             u = u01_from_bits(o[2]);
+            if ((F & SVPF_F_STUDENT_STATE) && P.nu_state <= 30.0f) {
+                // Student-t state noise (src/svpf_opt_kernels.cu:31-53): z sqrt(k / chi2_k)
+                const int k = max((int)(P.nu_state + 0.5f), 3);
+                float chi2 = 0.0f;
+                for (int c = 0; c < k; ++c) {
+                    const float w = rng.normal((uint32_t)i, (uint32_t)S.timestep, SVPF_PURPOSE_STATE_T0 + (uint32_t)c);
+                    chi2 = fmaf(w, w, chi2);
+                }
+                z *= sqrtf((float)k / (chi2 + 1e-8f));
+            }
         } else {
             z = D.z_pred[(size_t)t * n + i];
             if (D.u_pred) u = D.u_pred[(size_t)t * n + i];
@@ -707,13 +719,17 @@ __global__ __launch_bounds__(kGB) void grid_means(const svpf_fparams* __restrict
     mm[i] = g->sq * (mu + P.rho * (hp[i] - mu) + P.gamma * g->y_prev / (expf(hp[i] * 0.5f) + 1e-8f));
 }
 
-// gradient of rows r = blockIdx.x * (kGB/CS) + threadIdx.x / CS; CS lanes split the components
-template <int CS>
+// gradient of rows r = blockIdx.x * (kGB/CS) + threadIdx.x / CS; CS lanes split the components.
+// ST: the Student-t state prior (src/svpf_opt_kernels.cu:406-440) in units scaled by
+// 1/sqrt(nu_s sigma^2): r_i = exp2(a (log2 q_min - log2 q_i)), q = 1 + dd^2, a = (nu_s + 1)/2.
+// save (Heun): the positions at the iteration start are copied there.
+template <int CS, bool ST>
 __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restrict__ Pp,
                                                      const svpf_fstate* __restrict__ Sp,
                                                      const GridScalars* __restrict__ g, const float* __restrict__ h,
                                                      const float* __restrict__ mm, float4* __restrict__ rec,
-                                                     float* __restrict__ lw, int it, int n, int npad) {
+                                                     float* __restrict__ lw, float* __restrict__ save, int it, int n,
+                                                     int npad) {
     __shared__ __align__(16) float tile[kMeanTile];
     if (it >= g->budget || g->stop) return;  // uniform across the grid
     const svpf_fparams& P = *Pp;
@@ -725,10 +741,14 @@ __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restr
     const float sq = g->sq, inv_s2 = g->inv_s2, rs = g->rs;
     const float hi = i < n ? h[i] : 0.f;
     const float hh = hi * sq;
+    const float a_t = 0.5f * (P.nu_state + 1.0f);
     // pass 1: nearest component (the reference's max shift); pass 2: sums
-    float off = 3.0e38f, sr = 0.f, srd = 0.f;
+    float off = 3.0e38f, sr = 0.f, srd = 0.f, shp = 0.f;
     for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) off = group_min<CS>(off);
+        if (pass == 1) {
+            off = group_min<CS>(off);
+            if (ST) off = a_t * __builtin_amdgcn_logf(1.0f + off);
+        }
         for (int j0 = 0; j0 < npad; j0 += kMeanTile) {
             const int len = min(kMeanTile, npad - j0);
             __syncthreads();
@@ -741,6 +761,18 @@ __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restr
                 const float d0 = hh - m.x, d1 = hh - m.y, d2 = hh - m.z, d3 = hh - m.w;
                 if (pass == 0) {
                     off = fminf(off, fminf(fminf(d0 * d0, d1 * d1), fminf(d2 * d2, d3 * d3)));
+                } else if (ST) {
+                    const float dd[4] = {d0, d1, d2, d3};
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const float q = fmaf(dd[c], dd[c], 1.0f);
+                        const float iq = __builtin_amdgcn_rcpf(q);
+                        const float e = __builtin_amdgcn_exp2f(fmaf(-a_t, __builtin_amdgcn_logf(q), off));
+                        const float u = e * iq;
+                        sr += e;
+                        srd = fmaf(u, dd[c], srd);
+                        shp = fmaf(u * iq, 2.0f - q, shp);
+                    }
                 } else {
                     const float e0 = __builtin_amdgcn_exp2f(fmaf(-d0, d0, off));
                     const float e1 = __builtin_amdgcn_exp2f(fmaf(-d1, d1, off));
@@ -754,9 +786,19 @@ __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restr
     }
     sr = group_sum<CS>(sr);
     srd = group_sum<CS>(srd);
+    if (ST) shp = group_sum<CS>(shp);
     if (i >= n || cs != 0) return;
+    if (save) save[i] = hi;
     const float nu = P.nu, ysq = g->y_t * g->y_t;
-    const float gp = -((srd * (inv_s2 / sq)) / (sr + 1e-8f));
+    float gp, hpr;
+    if (ST) {
+        const float w = 1.0f / (sr + 1e-8f), np1s = (P.nu_state + 1.0f) * sq;
+        gp = -np1s * srd * w;
+        hpr = -np1s * sq * shp * w;
+    } else {
+        gp = -((srd * (inv_s2 / sq)) / (sr + 1e-8f));
+        hpr = -inv_s2;
+    }
     const float A = ysq / (safe_expf(hi) + 1e-8f) / nu, opa = 1.0f + A;
     lw[i] = (F & SVPF_F_FAN) ? 0.0f : P.st_const - 0.5f * hi - (nu + 1.0f) * 0.5f * log1pf(fmaxf(A, -0.999f));
     const float gl = (F & SVPF_F_EXACT_GRAD) ? -0.5f + 0.5f * (nu + 1.0f) * A / opa - P.lik_offset
@@ -766,8 +808,9 @@ __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restr
     const float sgn2 = (F & SVPF_F_PAPER_SIGN) ? 2.0f : -2.0f;
     float sj = gg, cj = sgn2, cv = 1.0f;
     if (F & SVPF_F_NEWTON) {
-        cv = fminf(fmaxf(0.5f * (nu + 1.0f) * A / (opa * opa) + inv_s2, 0.1f), 100.0f);
-        if (!(F & SVPF_F_FULL_NEWTON)) {
+        cv = fminf(fmaxf(0.5f * (nu + 1.0f) * A / (opa * opa) - hpr, 0.1f), 100.0f);
+        // the Heun step takes the raw score (src/svpf_optimized_graph.cu:1147-1232)
+        if (!(F & SVPF_F_FULL_NEWTON) && !(F & SVPF_F_HEUN)) {
             sj = 0.7f * gg / cv;
             cj = sgn2 / cv;
         }
@@ -775,10 +818,26 @@ __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restr
     rec[i] = make_float4(hi * rs, sj, cj * rs, cv);
 }
 
-template <int CS, int MODE, bool KSD>
+// Langevin normal of particle i at Stein iteration it (counter generator or pre-drawn row)
+SVPF_DEV float grid_xi(const svpf_fdesc& D, const svpf_fstate& S, int i, int it, int t, int n) {
+    if (D.z_pred == nullptr) {
+        CounterRng rng{(uint32_t)D.seed, (uint32_t)(D.seed >> 32), D.filter_id};
+        return rng.normal((uint32_t)i, (uint32_t)S.timestep, SVPF_PURPOSE_SVLD0 + (uint32_t)it);
+    }
+    return (it < D.xi_rows) ? D.xi[((size_t)t * D.xi_rows + it) * n + i] : 0.0f;
+}
+
+// What a transport pass does with phi: PH_UPDATE the RMSProp step; the Heun
+// phases (src/svpf_optimized_graph.cu:1147-1232): PH_PRED stores phi1 and moves
+// h to the predictor h~ = clamp(h0 + eff phi1 / sqrt(v + e)) (no noise, v kept),
+// PH_PHI2 stores (phi1 + phi2) / 2, PH_KSD only the KSD partials.
+enum { PH_UPDATE = 0, PH_PRED = 1, PH_PHI2 = 2, PH_KSD = 3 };
+
+template <int CS, int MODE, bool KSD, int PH = PH_UPDATE>
 SVPF_DEV void grid_transport_rows(const svpf_fparams& P, const svpf_fstate& S, const svpf_fdesc& D,
                                   const GridScalars* g, const float4* __restrict__ rec, float* __restrict__ h,
-                                  float* __restrict__ v, float* __restrict__ part, float4* tile, int it, int t, int n) {
+                                  float* __restrict__ v, float* __restrict__ part, float4* tile, int it, int t, int n,
+                                  const float* __restrict__ ho = nullptr, float* __restrict__ phib = nullptr) {
     const int i = blockIdx.x * (kGB / CS) + threadIdx.x / CS, cs = threadIdx.x % CS;
     const float ib = g->ib, rs = g->rs, two_ib = 2.0f * ib;
     const float4 own = i < n ? rec[i] : make_float4(0.f, 0.f, 0.f, 1.f);
@@ -830,21 +889,19 @@ SVPF_DEV void grid_transport_rows(const svpf_fparams& P, const svpf_fstate& S, c
         bool last;
         iter_schedule(P, S, g->budget, it, beta, last);
         const float eff = P.base_step * sqrtf(beta);
-        const float temp = (P.flags & SVPF_F_SVLD) ? P.temperature : 0.0f;
-        const float vn = P.rms_rho * v[i] + (1.0f - P.rms_rho) * p * p;
-        float diff = 0.0f;
-        if (temp > 1e-6f) {
-            float xi;
-            if (D.z_pred == nullptr) {
-                CounterRng rng{(uint32_t)D.seed, (uint32_t)(D.seed >> 32), D.filter_id};
-                xi = rng.normal((uint32_t)i, (uint32_t)S.timestep, SVPF_PURPOSE_SVLD0 + (uint32_t)it);
-            } else {
-                xi = (it < D.xi_rows) ? D.xi[((size_t)t * D.xi_rows + it) * n + i] : 0.0f;
-            }
-            diff = sqrtf(2.0f * eff * temp) * xi;
+        if (PH == PH_UPDATE) {
+            const float temp = (P.flags & SVPF_F_SVLD) ? P.temperature : 0.0f;
+            const float vn = P.rms_rho * v[i] + (1.0f - P.rms_rho) * p * p;
+            float diff = 0.0f;
+            if (temp > 1e-6f) diff = sqrtf(2.0f * eff * temp) * grid_xi(D, S, i, it, t, n);
+            v[i] = vn;
+            h[i] = clamp_logvol(h[i] + eff * p * rsqrtf(vn + P.rms_eps) + diff);
+        } else if (PH == PH_PRED) {
+            phib[i] = p;
+            h[i] = clamp_logvol(ho[i] + eff * p * rsqrtf(v[i] + P.rms_eps));
+        } else if (PH == PH_PHI2) {
+            phib[i] = 0.5f * (phib[i] + p);
         }
-        v[i] = vn;
-        h[i] = clamp_logvol(h[i] + eff * p * rsqrtf(vn + P.rms_eps) + diff);
     }
     if (KSD) {
         float* red = reinterpret_cast<float*>(tile);
@@ -882,6 +939,77 @@ __global__ __launch_bounds__(kGB) void grid_transport(const svpf_fparams* __rest
         if (ksd) grid_transport_rows<CS, MODE_PLAIN, true>(P, *Sp, *Dp, g, rec, h, v, part, tile, it, t, n);
         else grid_transport_rows<CS, MODE_PLAIN, false>(P, *Sp, *Dp, g, rec, h, v, part, tile, it, t, n);
     }
+}
+
+// The Heun passes of one Stein iteration: PH_PRED and PH_PHI2 with the raw-score
+// operator (plain, or full Newton's kernel-weighted curvature); PH_KSD the KSD at
+// the original positions with the predicted positions' scores, when this
+// iteration needs it.
+template <int CS, int PH>
+__global__ __launch_bounds__(kGB) void grid_heun_transport(const svpf_fparams* __restrict__ Pp,
+                                                           const svpf_fstate* __restrict__ Sp,
+                                                           const svpf_fdesc* __restrict__ Dp,
+                                                           const GridScalars* __restrict__ g,
+                                                           const float4* __restrict__ rec, float* __restrict__ h,
+                                                           float* __restrict__ v, float* __restrict__ part,
+                                                           const float* __restrict__ ho, float* __restrict__ phib,
+                                                           int it, int t, int n) {
+    __shared__ __align__(16) float4 tile[kRecTile];
+    if (it >= g->budget || g->stop) return;
+    const svpf_fparams& P = *Pp;
+    const uint32_t F = P.flags;
+    float beta;
+    bool last;
+    iter_schedule(P, *Sp, g->budget, it, beta, last);
+    if (PH == PH_KSD) {
+        if (!(last || (F & SVPF_F_KSD_EARLY_STOP))) return;
+        grid_transport_rows<CS, MODE_PLAIN, true, PH_KSD>(P, *Sp, *Dp, g, rec, h, v, part, tile, it, t, n, ho, phib);
+    } else if ((F & SVPF_F_NEWTON) && (F & SVPF_F_FULL_NEWTON)) {
+        grid_transport_rows<CS, MODE_FULL, false, PH>(P, *Sp, *Dp, g, rec, h, v, part, tile, it, t, n, ho, phib);
+    } else {
+        grid_transport_rows<CS, MODE_PLAIN, false, PH>(P, *Sp, *Dp, g, rec, h, v, part, tile, it, t, n, ho, phib);
+    }
+}
+
+// Heun: the records' positions back at the iteration start (the KSD pass's columns)
+__global__ __launch_bounds__(kGB) void grid_heun_ksd_records(const svpf_fparams* __restrict__ Pp,
+                                                             const svpf_fstate* __restrict__ Sp,
+                                                             const GridScalars* __restrict__ g,
+                                                             const float* __restrict__ ho, float4* __restrict__ rec,
+                                                             int it, int n) {
+    if (it >= g->budget || g->stop) return;
+    float beta;
+    bool last;
+    iter_schedule(*Pp, *Sp, g->budget, it, beta, last);
+    if (!(last || (Pp->flags & SVPF_F_KSD_EARLY_STOP))) return;
+    const int i = blockIdx.x * kGB + threadIdx.x;
+    if (i < n) rec[i].x = ho[i] * g->rs;
+}
+
+// Heun corrector: from the iteration-start positions with the mean of both operators,
+// the RMSProp update and the Langevin noise
+__global__ __launch_bounds__(kGB) void grid_heun_correct(const svpf_fparams* __restrict__ Pp,
+                                                         const svpf_fstate* __restrict__ Sp,
+                                                         const svpf_fdesc* __restrict__ Dp,
+                                                         const GridScalars* __restrict__ g,
+                                                         const float* __restrict__ ho, const float* __restrict__ phib,
+                                                         float* __restrict__ h, float* __restrict__ v, int it, int t,
+                                                         int n) {
+    if (it >= g->budget || g->stop) return;
+    const int i = blockIdx.x * kGB + threadIdx.x;
+    if (i >= n) return;
+    const svpf_fparams& P = *Pp;
+    float beta;
+    bool last;
+    iter_schedule(P, *Sp, g->budget, it, beta, last);
+    const float eff = P.base_step * sqrtf(beta);
+    const float temp = (P.flags & SVPF_F_SVLD) ? P.temperature : 0.0f;
+    const float p = phib[i];
+    const float vn = P.rms_rho * v[i] + (1.0f - P.rms_rho) * p * p;
+    float diff = 0.0f;
+    if (temp > 1e-6f) diff = sqrtf(2.0f * eff * temp) * grid_xi(*Dp, *Sp, i, it, t, n);
+    v[i] = vn;
+    h[i] = clamp_logvol(ho[i] + eff * p * rsqrtf(vn + P.rms_eps) + diff);
 }
 
 __global__ __launch_bounds__(kGB) void grid_ksd(const svpf_fparams* __restrict__ Pp,
@@ -1032,6 +1160,8 @@ struct GridWs {
     uint32_t* gcount;
     uint32_t* mpart;
     float* mcand;
+    float* ho;    // Heun: positions at the iteration start
+    float* phib;  // Heun: phi1, then (phi1 + phi2) / 2
 };
 
 size_t grid_ws_layout(int n, float* base, GridWs* w) {
@@ -1060,6 +1190,10 @@ size_t grid_ws_layout(int n, float* base, GridWs* w) {
     o += r4(2 * (size_t)kMaxMG * kMW);
     const size_t o_mc = o;
     o += kMedCand;
+    const size_t o_ho = o;
+    o += r4(n);
+    const size_t o_phib = o;
+    o += r4(n);
     static_assert(sizeof(GridScalars) <= 32 * sizeof(float) && 2 * sizeof(MedState) <= 28 * sizeof(float), "ws slots");
     if (w) {
         w->rec = reinterpret_cast<float4*>(base + o_rec);
@@ -1074,6 +1208,8 @@ size_t grid_ws_layout(int n, float* base, GridWs* w) {
         w->gcount = reinterpret_cast<uint32_t*>(base + o_ms + 28);
         w->mpart = reinterpret_cast<uint32_t*>(base + o_mp);
         w->mcand = base + o_mc;
+        w->ho = base + o_ho;
+        w->phib = base + o_phib;
     }
     return o;
 }
@@ -1108,18 +1244,37 @@ extern "C" hipError_t svpf_grid_init(void) {
     return done;
 }
 
+template <int CS, bool ST>
+void grid_iteration(const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, const GridWs& W, float* h, float* v,
+                    float* lw, bool heun, int G, int nb, int it, int t, int n, int npad, hipStream_t stream) {
+    if (!heun) {
+        grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, nullptr, it, n, npad);
+        grid_transport<CS><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, it, t, n);
+        return;
+    }
+    grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, W.ho, it, n, npad);
+    grid_heun_transport<CS, PH_PRED><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, W.ho, W.phib, it, t, n);
+    grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, nullptr, it, n, npad);
+    grid_heun_transport<CS, PH_PHI2><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, W.ho, W.phib, it, t, n);
+    grid_heun_ksd_records<<<nb, kGB, 0, stream>>>(P, S, W.gsc, W.ho, W.rec, it, n);
+    grid_heun_transport<CS, PH_KSD><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, W.ho, W.phib, it, t, n);
+    grid_heun_correct<<<nb, kGB, 0, stream>>>(P, S, D, W.gsc, W.ho, W.phib, h, v, it, t, n);
+}
+
 #define SVPF_GRID_ITER(CSV)                                                                                       \
     case CSV:                                                                                                     \
-        grid_gradient<CSV><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, it, n, npad);                   \
-        grid_transport<CSV><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, it, t, n);                 \
+        if (student) grid_iteration<CSV, true>(P, S, D, W, h, v, lw, heun, G, nb, it, t, n, npad, stream);       \
+        else grid_iteration<CSV, false>(P, S, D, W, h, v, lw, heun, G, nb, it, t, n, npad, stream);              \
         break;
 
 // Ticks [t0, t1) of one filter on `stream`: per tick 10 + med_rounds(n) + 3 stein_max launches
-// (stein_max bounds the device-side Stein budget).  h, hp, v, lw: the filter's
-// particle arrays [n]; ws: svpf_grid_workspace_floats(n) floats.
+// (9 per iteration with the Heun step; stein_max bounds the device-side Stein budget).
+// h, hp, v, lw: the filter's particle arrays [n]; ws: svpf_grid_workspace_floats(n) floats;
+// ext: SVPF_GRID_EXT_HEUN / SVPF_GRID_EXT_STUDENT as the filter's flags have them.
 extern "C" hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, float* h,
-                                       float* hp, float* v, float* lw, float* ws, int n, int stein_max, int t0,
-                                       int t1, hipStream_t stream) {
+                                       float* hp, float* v, float* lw, float* ws, int n, int stein_max, unsigned ext,
+                                       int t0, int t1, hipStream_t stream) {
+    const bool heun = (ext & SVPF_GRID_EXT_HEUN) != 0, student = (ext & SVPF_GRID_EXT_STUDENT) != 0;
     if (n < 2 || n > kMaxN || stein_max < 0 || t0 < 0 || t1 < t0) return hipErrorInvalidValue;
     hipError_t e = svpf_grid_init();
     if (e != hipSuccess) return e;

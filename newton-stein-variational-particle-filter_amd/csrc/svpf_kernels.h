@@ -83,8 +83,9 @@ extern "C" {
 size_t svpf_grid_workspace_floats(int n);
 size_t svpf_grid_lds_bytes(int n);
 hipError_t svpf_grid_init(void);
+enum { SVPF_GRID_EXT_HEUN = 1, SVPF_GRID_EXT_STUDENT = 2 };
 hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, float* h, float* hp,
-                            float* v, float* lw, float* ws, int n, int stein_max, int t0, int t1,
+                            float* v, float* lw, float* ws, int n, int stein_max, unsigned ext, int t0, int t1,
                             hipStream_t stream);
 #ifdef __cplusplus
 }
