@@ -189,7 +189,8 @@ def time_fused_v2(torch, n=512, T=1000, warm=100):
 def time_extended(torch, B=512, n=512, warm=2, T=10):
     """The batched kernel's extended instantiation: B filters of n particles
     with the plain configuration, with the Heun step, with the Student-t state
-    (nu_s = 5) and with both, each as one batched run (filter-steps/s)."""
+    (nu_s = 5) and with both, each as one batched run (filter-steps/s); then one
+    N=4096 filter on the grid path in the same four variants (device loop, us/step)."""
     from svpf_amd import config, data
 
     y = np.stack([data.sv_series(warm + T, seed=b)[0] for b in range(B)])
@@ -201,6 +202,25 @@ def time_extended(torch, B=512, n=512, warm=2, T=10):
         cfgs = [config.plain_config(n=n, **feats) for _ in range(B)]
         wall, _, _ = time_batched(torch, cfgs, y, h0s, warm, T)
         out[name + "_filter_steps_per_s"] = B * T / wall
+    # one filter on the grid path (device loop): plain, Heun, Student-t state, both
+    from svpf_amd.filter import Filter
+
+    N, TT = 4096, 200
+    ys, _ = data.sv_series(TT + 20, seed=7)
+    dy = torch.tensor(ys, device="cuda")
+    for name, feats in (("plain", {}), ("heun", dict(use_heun=1)),
+                        ("student_t_state", dict(use_student_t_state=1, nu_state=5.0)),
+                        ("heun_student_t_state", dict(use_heun=1, use_student_t_state=1, nu_state=5.0))):
+        f = Filter(config.plain_config(n=N, **feats))
+        f.set_particles(data.initial_particles(N, -1.0, 0.97, 0.15, seed=8))
+        assert f.path() == "grid"
+        f.run_device(dy[:20])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f.run_device(dy[20:])
+        torch.cuda.synchronize()
+        out[f"grid_n{N}_{name}_us_per_step"] = (time.perf_counter() - t0) / TT * 1e6
+        f.close()
     return out
 
 
