@@ -11,7 +11,7 @@
  *     svpf_last_error(state); svpf_create returns NULL on any failure;
  *   - features the reference offers but this build does not implement
  *     (Student-t state noise, Heun integration, the two-factor branch) are
- *     rejected with SVPF_ERR_UNSUPPORTED instead of being silently ignored;
+ *     rejected with SVPF_RC_UNSUPPORTED instead of being silently ignored;
  *   - svpf_step_graph really replays a captured HIP graph per tick;
  *   - three configuration fields: bandwidth_mode, use_linear_beta,
  *     use_ksd_early_stop (defaults reproduce the reference).
@@ -24,6 +24,7 @@
 #include <stdbool.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <stdio.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -55,14 +56,15 @@ extern "C" {
 #define SVPF_BANDWIDTH_MEDIAN 1        /* ell^2 = lower median of pairwise r^2 */
 #define SVPF_BANDWIDTH_SILVERMAN 2     /* src/svpf_fully_fused_v2.cu:397-424 */
 
-/* Status codes (svpf_last_error). */
-#define SVPF_OK 0
-#define SVPF_ERR_INVALID_ARGUMENT 1
-#define SVPF_ERR_UNSUPPORTED 2
-#define SVPF_ERR_HIP 3
-#define SVPF_ERR_NOT_INITIALIZED 4
-#define SVPF_ERR_OUT_OF_MEMORY 5
-#define SVPF_ERR_TIMEOUT 6 /* a cooperative kernel's cross-workgroup exchange timed out */
+/* Return codes (svpf_last_error).  Prefixed SVPF_RC_ so that the SVPFStatus
+ * names of svpf_diagnostics.h (SVPF_OK, SVPF_ERR_*) can be included beside them. */
+#define SVPF_RC_OK 0
+#define SVPF_RC_INVALID_ARGUMENT 1
+#define SVPF_RC_UNSUPPORTED 2
+#define SVPF_RC_HIP 3
+#define SVPF_RC_NOT_INITIALIZED 4
+#define SVPF_RC_OUT_OF_MEMORY 5
+#define SVPF_RC_TIMEOUT 6 /* a cooperative kernel's cross-workgroup exchange timed out */
 
 /* include/svpf.cuh:257-262 */
 typedef struct {
@@ -166,7 +168,7 @@ typedef struct {
     float ksd_prev;
     int stein_steps_used;
 
-    int use_student_t_state; /* single-workgroup kernel only (n <= its capacity); else SVPF_ERR_UNSUPPORTED */
+    int use_student_t_state; /* single-workgroup kernel only (n <= its capacity); else SVPF_RC_UNSUPPORTED */
     float nu_state;
 
     int use_smoothing;
@@ -179,8 +181,8 @@ typedef struct {
 
     int use_persistent_kernel; /* 1: cooperative multi-workgroup kernel (default for n >= 256) */
     int persistent_kernel_supported;
-    int use_heun;      /* single-workgroup kernel only (n <= its capacity); else SVPF_ERR_UNSUPPORTED */
-    int use_two_factor; /* rejected: SVPF_ERR_UNSUPPORTED */
+    int use_heun;      /* single-workgroup kernel only (n <= its capacity); else SVPF_RC_UNSUPPORTED */
+    int use_two_factor; /* rejected: SVPF_RC_UNSUPPORTED */
 
     /* extensions (defaults reproduce the reference) */
     int bandwidth_mode;
@@ -236,8 +238,8 @@ float svpf_get_ess(const SVPFState* state);
 /* src/svpf_optimized_graph.cu:1490-1493 (defined there, never declared) */
 void svpf_get_ksd_stats(const SVPFState* state, float* ksd_out, int* steps_used_out);
 
-/* No reference counterpart: status of the last call on this state (SVPF_OK ...)
- * and a readable description (the HIP error string for SVPF_ERR_HIP). */
+/* No reference counterpart: status of the last call on this state (SVPF_RC_OK ...)
+ * and a readable description (the HIP error string for SVPF_RC_HIP). */
 int svpf_last_error(const SVPFState* state);
 const char* svpf_last_error_string(const SVPFState* state);
 
@@ -289,7 +291,7 @@ int svpf_set_coop_spin_limit(SVPFState* state, int polls);
  * features in use -- cooperative kernel for 256 <= n < 1024 where it applies,
  * grid path from n = 1024 up and for 256 <= n < 1024 otherwise, one workgroup
  * below 256; SVPF_GRID_ON forces the grid path; SVPF_GRID_OFF forbids it
- * (SVPF_ERR_UNSUPPORTED when n_particles exceeds the single-workgroup
+ * (SVPF_RC_UNSUPPORTED when n_particles exceeds the single-workgroup
  * capacity).  svpf_get_path returns the SVPF_PATH_* of the next step, -1 for a
  * bad state. */
 #define SVPF_PATH_SINGLE 0
@@ -342,7 +344,7 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
  * log-likelihood of every tick, the median bandwidth of every tick and a
  * checkpoint (x, v) every checkpoint_every ticks; svpf_diff_backward takes
  * dL/d pred_ll and writes dL/d theta (and dL/d h0), recomputing each
- * checkpoint segment into the tape.  Both return SVPF_OK or an SVPF_ERR_*
+ * checkpoint segment into the tape.  Both return SVPF_RC_OK or an SVPF_ERR_*
  * code.  svpf_diff_tape_floats is the tape size per filter.  With the optional
  * svpf_ddesc.drift ([T][S][n] floats per filter) the forward stores every
  * iteration's Stein drift and the backward replays positions from it instead
@@ -377,6 +379,95 @@ int svpf_tile_occupancy_query(int tile, int block_size, int nmax);
 size_t svpf_lds_bytes_query(int nmax);
 /* Library version string. */
 const char* svpf_version(void);
+
+/* ---- Score diagnostics beside the step (include/svpf.cuh:895-1213) ----
+ * Posterior-weighted scores of the SV model read from the filter after a
+ * step: d/dnu of the Student-t observation density at the particles h, and
+ * d/dsigma of the transition density of h_pred given h_prev, each with its
+ * weighted mean z^2 or eps^2/sigma^2 (csrc/svpf_diag.hip, one reduction
+ * kernel on the state's stream and a synchronous read-back).  The host side
+ * keeps the reference's EMA smoothing, CSV log and shock state machine.  The
+ * reference declares svpf_compute_all_gradients (include/svpf.cuh:1102) and
+ * svpf_test_nu_gradient_synthetic without defining them in the library; they
+ * are not part of this ABI either. */
+
+/* include/svpf.cuh:895-929 */
+typedef struct {
+    float* d_nu_grad;   /* device: weighted mean nu score */
+    float* d_z_sq_mean; /* device: weighted mean z^2 */
+    float* d_mu_grad;   /* unused (null), as there */
+    float* d_rho_grad;
+    float* d_sigma_grad;
+    float* d_fisher;
+    float* d_fisher_inv;
+    float nu_gradient_ema; /* EMA, alpha 0.1 */
+    float z_sq_ema;
+    float mu_gradient_ema;
+    float rho_gradient_ema;
+    float sigma_gradient_ema;
+    int shock_state; /* 0 calm, 1 shock, 2 recovery */
+    int ticks_in_state;
+    float shock_threshold;         /* z^2 entering SHOCK (9) */
+    int shock_duration;            /* ticks in SHOCK (20) */
+    int recovery_duration;         /* ticks in RECOVERY before CALM (50) */
+    float recovery_exit_threshold; /* z^2 below which RECOVERY may end (4) */
+    bool enable_logging;
+    FILE* log_file;
+    bool initialized;
+} SVPFGradientDiagnostics;
+
+/* include/svpf.cuh:942-947: mu, rho = tanh(eta), sigma = exp(kappa), nu = 2 + exp(kappa_nu) */
+typedef struct {
+    float mu;
+    float eta;
+    float kappa;
+    float kappa_nu;
+} SVPFThetaUnconstrained;
+
+/* include/svpf.cuh:958-981 */
+typedef struct {
+    SVPFThetaUnconstrained theta;
+    float F[4][4];
+    float F_ema_decay;
+    float F_reg;
+    float base_lr;
+    float lr_shock_mult;
+    float grad_clip;
+    float prior_weight;
+    SVPFThetaUnconstrained theta_prior;
+    int warmup_ticks;
+    bool learning_enabled;
+} SVPFNaturalGradientTuner;
+
+/* include/svpf.cuh:994; NULL on allocation failure */
+SVPFGradientDiagnostics* svpf_gradient_diagnostic_create(bool enable_logging, const char* log_path);
+/* include/svpf.cuh:999 */
+void svpf_gradient_diagnostic_destroy(SVPFGradientDiagnostics* diag);
+/* include/svpf.cuh:1019-1026: the nu score at y_t over the current particles,
+ * weighted by the log-weights, into diag's device buffers and EMAs (and its log) */
+void svpf_compute_nu_diagnostic(SVPFState* state, SVPFGradientDiagnostics* diag, float y_t, int timestep,
+                                float* nu_grad_out, float* z_sq_mean_out);
+/* include/svpf.cuh:1034-1039: the same without diagnostic state (the state's
+ * own scratch, no allocation) */
+void svpf_compute_nu_diagnostic_simple(SVPFState* state, float y_t, float* nu_grad_out, float* z_sq_mean_out);
+/* include/svpf.cuh:1062-1067: the sigma score over h_pred given h_prev, at the
+ * state's effective sigma (params->sigma_z when that is below 1e-6) */
+void svpf_compute_sigma_diagnostic_simple(SVPFState* state, const SVPFParams* params, float* sigma_grad_out,
+                                          float* eps_sq_norm_out);
+/* include/svpf.cuh:1074-1077: device-to-device copy of h on the state's stream */
+void svpf_snapshot_particles(SVPFState* state, float* d_h_buffer);
+/* include/svpf.cuh:1125-1143 */
+void svpf_update_shock_state(SVPFGradientDiagnostics* diag, float z_sq);
+int svpf_get_shock_state(const SVPFGradientDiagnostics* diag);
+bool svpf_should_learn(const SVPFGradientDiagnostics* diag);
+float svpf_get_lr_multiplier(const SVPFGradientDiagnostics* diag, float lr_shock_mult);
+/* include/svpf.cuh:1157-1198 (the update is the reference's placeholder: it
+ * returns the current parameters) */
+SVPFNaturalGradientTuner* svpf_tuner_create(const SVPFParams* params, float nu, float base_lr, float prior_weight);
+void svpf_tuner_destroy(SVPFNaturalGradientTuner* tuner);
+void svpf_tuner_update(SVPFNaturalGradientTuner* tuner, const SVPFGradientDiagnostics* diag, SVPFParams* params_out,
+                       float* nu_out);
+void svpf_tuner_get_params(const SVPFNaturalGradientTuner* tuner, SVPFParams* params_out, float* nu_out);
 
 /* include/svpf.cuh:1219-1256 */
 static inline float svpf_constrain_rho(float eta) { return tanhf(eta); }

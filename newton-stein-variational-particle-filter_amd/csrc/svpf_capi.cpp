@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "../../include/svpf.h"
+#include "svpf_internal.h"
 #include "svpf_kernels.h"
 #include "svpf_types.h"
 
@@ -31,12 +32,14 @@ constexpr int kBlockFstate = 256;
 constexpr int kBlockY = 384;      // 4 floats
 constexpr int kBlockOut = 400;    // 8 floats
 constexpr int kBlockDesc = 448;   // svpf_fdesc
+constexpr int kBlockDiag = 960;   // diagnostics scratch (16 floats)
 constexpr int kBlockBytes = 1024;
 constexpr int kUploadBytes = kBlockDesc + (int)sizeof(svpf_fdesc);
 static_assert(sizeof(svpf_fparams) == 256, "fparams layout");
 static_assert(sizeof(svpf_fstate) == 128, "fstate layout");
 static_assert(SVPF_TILE_EXT == SVPF_TILE_EXTENDED, "extended tile bit");
-static_assert(kBlockDesc + sizeof(svpf_fdesc) <= (size_t)kBlockBytes, "block layout");
+static_assert(kBlockDesc + sizeof(svpf_fdesc) <= (size_t)kBlockDiag, "block layout");
+static_assert(kBlockDiag + kDiagScratchFloats * sizeof(float) <= (size_t)kBlockBytes, "block layout");
 
 }  // namespace
 
@@ -62,8 +65,8 @@ struct svpf_impl {
     int xi_rows = 0, t_base = 0;
     uint64_t seed = 42;
     bool initialized = false;
-    int last_error = SVPF_OK;
-    hipError_t last_hip = hipSuccess;  // HIP status behind the last SVPF_ERR_HIP
+    int last_error = SVPF_RC_OK;
+    hipError_t last_hip = hipSuccess;  // HIP status behind the last SVPF_RC_HIP
     int last_status = 0;               // kernel-reported status behind the last kernel error
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -329,13 +332,13 @@ void fill_desc(SVPFState* s, svpf_fdesc* d, const float* y, float* out, int tick
 }
 
 int check_supported(SVPFState* s) {
-    if (s->use_two_factor) return SVPF_ERR_UNSUPPORTED;
-    if (s->n_particles != s->impl->n) return SVPF_ERR_INVALID_ARGUMENT;
-    if (s->bandwidth_mode < 0 || s->bandwidth_mode > 2) return SVPF_ERR_INVALID_ARGUMENT;
-    if (s->n_anneal_steps < 1 && s->use_annealing) return SVPF_ERR_INVALID_ARGUMENT;
+    if (s->use_two_factor) return SVPF_RC_UNSUPPORTED;
+    if (s->n_particles != s->impl->n) return SVPF_RC_INVALID_ARGUMENT;
+    if (s->bandwidth_mode < 0 || s->bandwidth_mode > 2) return SVPF_RC_INVALID_ARGUMENT;
+    if (s->n_anneal_steps < 1 && s->use_annealing) return SVPF_RC_INVALID_ARGUMENT;
     if (s->stein_max_steps < 0 || s->stein_min_steps < 0 || s->stein_min_steps > s->stein_max_steps)
-        return SVPF_ERR_INVALID_ARGUMENT;
-    return SVPF_OK;
+        return SVPF_RC_INVALID_ARGUMENT;
+    return SVPF_RC_OK;
 }
 
 // The cooperative multi-workgroup kernel serves one filter when the state asks
@@ -456,11 +459,11 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     if (!s || !s->impl) return;
     svpf_impl* I = s->impl;
     if (!p) {
-        set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+        set_err(s, SVPF_RC_INVALID_ARGUMENT);
         return;
     }
     if (!I->initialized) {
-        set_err(s, SVPF_ERR_NOT_INITIALIZED);
+        set_err(s, SVPF_RC_NOT_INITIALIZED);
         return;
     }
     int rc = check_supported(s);
@@ -533,30 +536,30 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     }
     if (e != hipSuccess) {
         I->last_hip = e;
-        set_err(s, SVPF_ERR_HIP);
+        set_err(s, SVPF_RC_HIP);
         return;
     }
     if (I->hs()->status) {
         I->last_status = I->hs()->status;
-        set_err(s, I->hs()->status == 2 ? SVPF_ERR_TIMEOUT : SVPF_ERR_UNSUPPORTED);
+        set_err(s, I->hs()->status == 2 ? SVPF_RC_TIMEOUT : SVPF_RC_UNSUPPORTED);
         return;
     }
     unpack_fstate(s, I->hs());
     finish_tick(s, p, y_t, I->hout(), ll, vol, hm);
-    set_err(s, SVPF_OK);
+    set_err(s, SVPF_RC_OK);
 }
 
 // T ticks in one launch; records land in I->dseq_out[T][8].
 int run_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, float y_prev0) {
     svpf_impl* I = s->impl;
-    if (!I->initialized) return SVPF_ERR_NOT_INITIALIZED;
+    if (!I->initialized) return SVPF_RC_NOT_INITIALIZED;
     int rc = check_supported(s);
     if (rc) return rc;
-    if (T <= 0) return SVPF_OK;
+    if (T <= 0) return SVPF_RC_OK;
     if (I->dseq_cap < T) {
         if (I->dseq_out) hipFree(I->dseq_out);
         I->dseq_out = nullptr;
-        if (hipMalloc(&I->dseq_out, (size_t)T * SVPF_OUTW * sizeof(float)) != hipSuccess) return SVPF_ERR_OUT_OF_MEMORY;
+        if (hipMalloc(&I->dseq_out, (size_t)T * SVPF_OUTW * sizeof(float)) != hipSuccess) return SVPF_RC_OUT_OF_MEMORY;
         I->dseq_cap = T;
     }
     pack_params(s, p, I->hp());
@@ -582,14 +585,14 @@ int run_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, flo
     }
     if (e != hipSuccess) {
         I->last_hip = e;
-        return SVPF_ERR_HIP;
+        return SVPF_RC_HIP;
     }
     if (I->hs()->status) {
         I->last_status = I->hs()->status;
-        return I->hs()->status == 2 ? SVPF_ERR_TIMEOUT : SVPF_ERR_UNSUPPORTED;
+        return I->hs()->status == 2 ? SVPF_RC_TIMEOUT : SVPF_RC_UNSUPPORTED;
     }
     unpack_fstate(s, I->hs());
-    return SVPF_OK;
+    return SVPF_RC_OK;
 }
 
 }  // namespace
@@ -753,7 +756,7 @@ void svpf_initialize(SVPFState* s, const SVPFParams* p, unsigned long long seed)
     if (!s || !s->impl) return;
     svpf_impl* I = s->impl;
     if (!p) {
-        set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+        set_err(s, SVPF_RC_INVALID_ARGUMENT);
         return;
     }
     s->timestep = 0;
@@ -768,7 +771,7 @@ void svpf_initialize(SVPFState* s, const SVPFParams* p, unsigned long long seed)
     hipError_t e = svpf_launch_init(I->d_h(), I->d_hp(), I->d_v(), I->n, p->mu, sd, seed, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
-        set_err(s, SVPF_ERR_HIP);
+        set_err(s, SVPF_RC_HIP);
         return;
     }
     s->guide_initialized = 0;
@@ -784,7 +787,7 @@ void svpf_initialize(SVPFState* s, const SVPFParams* p, unsigned long long seed)
     I->med_r2 = 0.0f;
     svpf_graph_invalidate(s);
     I->initialized = true;
-    set_err(s, SVPF_OK);
+    set_err(s, SVPF_RC_OK);
 }
 
 void svpf_step_graph(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float* ll, float* vol, float* hm) {
@@ -828,24 +831,24 @@ void svpf_run_sequence(SVPFState* s, const float* obs, int T, const SVPFParams* 
     if (!s || !s->impl) return;
     svpf_impl* I = s->impl;
     if (!obs || !p || T < 0) {
-        set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+        set_err(s, SVPF_RC_INVALID_ARGUMENT);
         return;
     }
     if (T == 0) {
-        set_err(s, SVPF_OK);
+        set_err(s, SVPF_RC_OK);
         return;
     }
     if (I->dseq_y_cap < T) {
         if (I->dseq_y) hipFree(I->dseq_y);
         I->dseq_y = nullptr;
         if (hipMalloc(&I->dseq_y, (size_t)T * sizeof(float)) != hipSuccess) {
-            set_err(s, SVPF_ERR_OUT_OF_MEMORY);
+            set_err(s, SVPF_RC_OUT_OF_MEMORY);
             return;
         }
         I->dseq_y_cap = T;
     }
     if (hipMemcpy(I->dseq_y, obs, (size_t)T * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
-        set_err(s, SVPF_ERR_HIP);
+        set_err(s, SVPF_RC_HIP);
         return;
     }
     int t0 = s->timestep;
@@ -856,7 +859,7 @@ void svpf_run_sequence(SVPFState* s, const float* obs, int T, const SVPFParams* 
     }
     std::vector<float> rec((size_t)T * SVPF_OUTW);
     if (hipMemcpy(rec.data(), I->dseq_out, rec.size() * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
-        set_err(s, SVPF_ERR_HIP);
+        set_err(s, SVPF_RC_HIP);
         return;
     }
     int tend = s->timestep;
@@ -866,7 +869,7 @@ void svpf_run_sequence(SVPFState* s, const float* obs, int T, const SVPFParams* 
         finish_tick(s, p, obs[t], r, ll ? &ll[t] : nullptr, vol ? &vol[t] : nullptr, nullptr);
     }
     s->timestep = tend;
-    set_err(s, SVPF_OK);
+    set_err(s, SVPF_RC_OK);
 }
 
 void svpf_run_sequence_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, float* d_ll,
@@ -874,7 +877,7 @@ void svpf_run_sequence_device(SVPFState* s, const float* d_obs, int T, const SVP
     if (!s || !s->impl) return;
     svpf_impl* I = s->impl;
     if (!d_obs || !p || T < 0) {
-        set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+        set_err(s, SVPF_RC_INVALID_ARGUMENT);
         return;
     }
     int rc = run_device(s, d_obs, T, p, 0.0f);
@@ -895,7 +898,7 @@ void svpf_run_sequence_device(SVPFState* s, const float* d_obs, int T, const SVP
         e = hipMemcpy(last, I->dseq_out + (size_t)(T - 1) * SVPF_OUTW, sizeof(last), hipMemcpyDeviceToHost);
         if (e == hipSuccess) std::memcpy(I->out_last, last, sizeof(last));
     }
-    set_err(s, e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP);
+    set_err(s, e == hipSuccess ? SVPF_RC_OK : SVPF_RC_HIP);
 }
 
 bool svpf_graph_is_captured(SVPFState* s) { return s && s->impl && s->impl->captured; }
@@ -914,7 +917,7 @@ void svpf_get_particles(const SVPFState* s, float* h_out) {
     if (!s || !s->impl || !h_out) return;
     hipStreamSynchronize(s->stream);
     hipError_t e = hipMemcpy(h_out, s->impl->d_h(), (size_t)s->impl->n * sizeof(float), hipMemcpyDeviceToHost);
-    s->impl->last_error = e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+    s->impl->last_error = e == hipSuccess ? SVPF_RC_OK : SVPF_RC_HIP;
 }
 
 void svpf_get_stats(const SVPFState* s, float* h_mean, float* h_std) {
@@ -938,7 +941,7 @@ float svpf_get_ess(const SVPFState* s) {
     std::vector<float> lw(n);
     hipStreamSynchronize(s->stream);
     if (hipMemcpy(lw.data(), s->impl->d_lw(), (size_t)n * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
-        s->impl->last_error = SVPF_ERR_HIP;
+        s->impl->last_error = SVPF_RC_HIP;
         return 0.0f;
     }
     float mx = lw[0];
@@ -958,25 +961,25 @@ void svpf_get_ksd_stats(const SVPFState* s, float* ksd_out, int* steps_used_out)
     if (steps_used_out) *steps_used_out = s->stein_steps_used;
 }
 
-int svpf_last_error(const SVPFState* s) { return (s && s->impl) ? s->impl->last_error : SVPF_ERR_INVALID_ARGUMENT; }
+int svpf_last_error(const SVPFState* s) { return (s && s->impl) ? s->impl->last_error : SVPF_RC_INVALID_ARGUMENT; }
 
 const char* svpf_last_error_string(const SVPFState* s) {
     if (!s || !s->impl) return "invalid state";
     const svpf_impl* I = s->impl;
     switch (I->last_error) {
-        case SVPF_OK: return "ok";
-        case SVPF_ERR_INVALID_ARGUMENT: return "invalid argument";
-        case SVPF_ERR_UNSUPPORTED: return "unsupported feature";
-        case SVPF_ERR_HIP: return hipGetErrorString(I->last_hip);
-        case SVPF_ERR_NOT_INITIALIZED: return "filter not initialized";
-        case SVPF_ERR_OUT_OF_MEMORY: return "out of memory";
-        case SVPF_ERR_TIMEOUT: return "cooperative exchange timed out";
+        case SVPF_RC_OK: return "ok";
+        case SVPF_RC_INVALID_ARGUMENT: return "invalid argument";
+        case SVPF_RC_UNSUPPORTED: return "unsupported feature";
+        case SVPF_RC_HIP: return hipGetErrorString(I->last_hip);
+        case SVPF_RC_NOT_INITIALIZED: return "filter not initialized";
+        case SVPF_RC_OUT_OF_MEMORY: return "out of memory";
+        case SVPF_RC_TIMEOUT: return "cooperative exchange timed out";
         default: return "unknown";
     }
 }
 
 int svpf_set_particles(SVPFState* s, const float* h, const float* h_prev, const float* v) {
-    if (!s || !s->impl || !h) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+    if (!s || !s->impl || !h) return set_err(s, SVPF_RC_INVALID_ARGUMENT);
     svpf_impl* I = s->impl;
     size_t nb = (size_t)I->n * sizeof(float);
     hipStreamSynchronize(s->stream);
@@ -986,24 +989,24 @@ int svpf_set_particles(SVPFState* s, const float* h, const float* h_prev, const 
         if (v) e = hipMemcpy(I->d_v(), v, nb, hipMemcpyHostToDevice);
         else e = hipMemset(I->d_v(), 0, nb);
     }
-    if (e != hipSuccess) return set_err(s, SVPF_ERR_HIP);
+    if (e != hipSuccess) return set_err(s, SVPF_RC_HIP);
     I->initialized = true;
-    return set_err(s, SVPF_OK);
+    return set_err(s, SVPF_RC_OK);
 }
 
 int svpf_get_rmsprop_state(const SVPFState* s, float* v_out) {
-    if (!s || !s->impl || !v_out) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!s || !s->impl || !v_out) return SVPF_RC_INVALID_ARGUMENT;
     hipStreamSynchronize(s->stream);
     return hipMemcpy(v_out, s->impl->d_v(), (size_t)s->impl->n * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess
-               ? SVPF_OK
-               : SVPF_ERR_HIP;
+               ? SVPF_RC_OK
+               : SVPF_RC_HIP;
 }
 
 int svpf_set_predrawn_noise(SVPFState* s, const float* z_pred, const float* u_pred, const float* xi, int xi_rows,
                             const float* u_rej, const float* z_rej, int t_base) {
-    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
     svpf_impl* I = s->impl;
-    if (z_pred && (!xi || xi_rows < 0)) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+    if (z_pred && (!xi || xi_rows < 0)) return set_err(s, SVPF_RC_INVALID_ARGUMENT);
     I->z_pred = z_pred;
     I->u_pred = z_pred ? u_pred : nullptr;
     I->xi = z_pred ? xi : nullptr;
@@ -1012,22 +1015,22 @@ int svpf_set_predrawn_noise(SVPFState* s, const float* z_pred, const float* u_pr
     I->z_rej = z_pred ? z_rej : nullptr;
     I->t_base = t_base;
     svpf_graph_invalidate(s);
-    return set_err(s, SVPF_OK);
+    return set_err(s, SVPF_RC_OK);
 }
 
 int svpf_set_scalar_state(SVPFState* s, float h_mean_prev, float bw_sq_ema, float return_ema, float return_var) {
-    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
     s->impl->h_mean_prev = h_mean_prev;
     s->impl->bw_sq_ema = bw_sq_ema;
     s->impl->return_ema = return_ema;
     s->impl->return_var = return_var;
-    return set_err(s, SVPF_OK);
+    return set_err(s, SVPF_RC_OK);
 }
 
 int svpf_get_last_outputs(const SVPFState* s, float out8[8]) {
-    if (!s || !s->impl || !out8) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!s || !s->impl || !out8) return SVPF_RC_INVALID_ARGUMENT;
     std::memcpy(out8, s->impl->out_last, 8 * sizeof(float));
-    return SVPF_OK;
+    return SVPF_RC_OK;
 }
 
 int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, float* d_pool_h, float* d_pool_hp,
@@ -1036,24 +1039,24 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
                      int t0, int t1, hipStream_t stream) {
     if (!d_params || !d_fstate || !d_desc || !d_pool_h || !d_pool_hp || !d_pool_v || !d_order || !group_start ||
         !group_tile || !group_bs || !group_grid || !group_nmax || n_groups < 0 || t1 < t0)
-        return SVPF_ERR_INVALID_ARGUMENT;
-    if (svpf_kernels_init() != hipSuccess) return SVPF_ERR_HIP;
+        return SVPF_RC_INVALID_ARGUMENT;
+    if (svpf_kernels_init() != hipSuccess) return SVPF_RC_HIP;
     int used = 0;
     for (int g = 0; g < n_groups; ++g) {
         int nf = group_start[g + 1] - group_start[g];
         if (nf <= 0) continue;
         int cap = svpf_tile_capacity(group_tile[g], group_bs[g]);
-        if (cap <= 0 || group_nmax[g] > cap || group_grid[g] <= 0) return SVPF_ERR_INVALID_ARGUMENT;
-        if (svpf_lds_bytes(group_nmax[g]) > 159 * 1024) return SVPF_ERR_UNSUPPORTED;
+        if (cap <= 0 || group_nmax[g] > cap || group_grid[g] <= 0) return SVPF_RC_INVALID_ARGUMENT;
+        if (svpf_lds_bytes(group_nmax[g]) > 159 * 1024) return SVPF_RC_UNSUPPORTED;
         ++used;
     }
-    if (used > kMaxForkStreams) return SVPF_ERR_INVALID_ARGUMENT;
+    if (used > kMaxForkStreams) return SVPF_RC_INVALID_ARGUMENT;
     // Tile groups run concurrently: fork from `stream` onto worker streams and
     // join back, so the call stays one ordered operation on `stream` (and is
     // capturable into a graph).
     ForkJoin* fj = used > 1 ? fork_join() : nullptr;
-    if (used > 1 && !fj) return SVPF_ERR_HIP;
-    if (fj && hipEventRecord(fj->fork, stream) != hipSuccess) return SVPF_ERR_HIP;
+    if (used > 1 && !fj) return SVPF_RC_HIP;
+    if (fj && hipEventRecord(fj->fork, stream) != hipSuccess) return SVPF_RC_HIP;
     int k = 0;
     for (int g = 0; g < n_groups; ++g) {
         int nf = group_start[g + 1] - group_start[g];
@@ -1061,36 +1064,36 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
         hipStream_t s = stream;
         if (fj) {
             s = fj->streams[k];
-            if (hipStreamWaitEvent(s, fj->fork, 0) != hipSuccess) return SVPF_ERR_HIP;
+            if (hipStreamWaitEvent(s, fj->fork, 0) != hipSuccess) return SVPF_RC_HIP;
         }
         hipError_t e = svpf_launch_batched(group_bs[g], group_tile[g], group_grid[g],
                                            static_cast<const svpf_fparams*>(d_params),
                                            static_cast<svpf_fstate*>(d_fstate), static_cast<const svpf_fdesc*>(d_desc),
                                            d_pool_h, d_pool_hp, d_pool_v, d_order + group_start[g], nf,
                                            d_queues ? d_queues + g : nullptr, group_nmax[g], t0, t1, s);
-        if (e != hipSuccess) return SVPF_ERR_HIP;
+        if (e != hipSuccess) return SVPF_RC_HIP;
         if (fj) {
-            if (hipEventRecord(fj->join[k], s) != hipSuccess) return SVPF_ERR_HIP;
-            if (hipStreamWaitEvent(stream, fj->join[k], 0) != hipSuccess) return SVPF_ERR_HIP;
+            if (hipEventRecord(fj->join[k], s) != hipSuccess) return SVPF_RC_HIP;
+            if (hipStreamWaitEvent(stream, fj->join[k], 0) != hipSuccess) return SVPF_RC_HIP;
         }
         ++k;
     }
-    return SVPF_OK;
+    return SVPF_RC_OK;
 }
 
 int svpf_set_coop_shape(SVPFState* s, int lanes_per_row, int xcd_stride) {
-    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
     svpf_impl* I = s->impl;
-    if (I->tile < 0) return set_err(s, SVPF_ERR_UNSUPPORTED);
+    if (I->tile < 0) return set_err(s, SVPF_RC_UNSUPPORTED);
     svpf_impl trial = {};
     trial.n = I->n;
-    if (!choose_coop_shape(&trial, lanes_per_row, xcd_stride)) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+    if (!choose_coop_shape(&trial, lanes_per_row, xcd_stride)) return set_err(s, SVPF_RC_INVALID_ARGUMENT);
     I->coop_cs = trial.coop_cs;
     I->coop_G = trial.coop_G;
     I->coop_stride = trial.coop_stride;
     I->coop_ok = true;
     svpf_graph_invalidate(s);
-    return set_err(s, SVPF_OK);
+    return set_err(s, SVPF_RC_OK);
 }
 
 int svpf_coop_launch_is_cooperative(const SVPFState* s) {
@@ -1099,21 +1102,21 @@ int svpf_coop_launch_is_cooperative(const SVPFState* s) {
 }
 
 int svpf_set_coop_spin_limit(SVPFState* s, int polls) {
-    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
-    if (polls < -(1 << 20) || polls > (1 << 24)) return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+    if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
+    if (polls < -(1 << 20) || polls > (1 << 24)) return set_err(s, SVPF_RC_INVALID_ARGUMENT);
     s->impl->coop_spin_limit = polls;
-    return set_err(s, SVPF_OK);
+    return set_err(s, SVPF_RC_OK);
 }
 
 int svpf_set_grid_path(SVPFState* s, int mode) {
-    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
     svpf_impl* I = s->impl;
     if (mode != SVPF_GRID_AUTO && mode != SVPF_GRID_OFF && mode != SVPF_GRID_ON)
-        return set_err(s, SVPF_ERR_INVALID_ARGUMENT);
+        return set_err(s, SVPF_RC_INVALID_ARGUMENT);
     if ((mode == SVPF_GRID_OFF && I->tile < 0) || (mode == SVPF_GRID_ON && !I->dgrid_ws))
-        return set_err(s, SVPF_ERR_UNSUPPORTED);
+        return set_err(s, SVPF_RC_UNSUPPORTED);
     I->grid_mode = mode;
-    return set_err(s, SVPF_OK);
+    return set_err(s, SVPF_RC_OK);
 }
 
 int svpf_get_path(const SVPFState* s) {
@@ -1122,12 +1125,12 @@ int svpf_get_path(const SVPFState* s) {
 }
 
 int svpf_get_coop_shape(const SVPFState* s, int* lanes_per_row, int* workgroups, int* xcd_stride) {
-    if (!s || !s->impl) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
     const svpf_impl* I = s->impl;
     if (lanes_per_row) *lanes_per_row = I->coop_ok ? I->coop_cs : 0;
     if (workgroups) *workgroups = I->coop_ok ? I->coop_G : 0;
     if (xcd_stride) *xcd_stride = I->coop_ok ? I->coop_stride : 0;
-    return SVPF_OK;
+    return SVPF_RC_OK;
 }
 
 size_t svpf_diff_tape_floats(int n_particles, int n_stein, int checkpoint_every) {
@@ -1136,19 +1139,19 @@ size_t svpf_diff_tape_floats(int n_particles, int n_stein, int checkpoint_every)
 }
 
 int svpf_diff_forward(const void* config, const void* d_desc, int B, hipStream_t stream) {
-    if (!config || !d_desc || B < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!config || !d_desc || B < 1) return SVPF_RC_INVALID_ARGUMENT;
     const svpf_dconfig* c = static_cast<const svpf_dconfig*>(config);
-    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_RC_INVALID_ARGUMENT;
     hipError_t e = svpf_launch_diff(0, c, static_cast<const svpf_ddesc*>(d_desc), B, stream);
-    return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+    return e == hipSuccess ? SVPF_RC_OK : SVPF_RC_HIP;
 }
 
 int svpf_diff_backward(const void* config, const void* d_desc, int B, hipStream_t stream) {
-    if (!config || !d_desc || B < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!config || !d_desc || B < 1) return SVPF_RC_INVALID_ARGUMENT;
     const svpf_dconfig* c = static_cast<const svpf_dconfig*>(config);
-    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_RC_INVALID_ARGUMENT;
     hipError_t e = svpf_launch_diff(1, c, static_cast<const svpf_ddesc*>(d_desc), B, stream);
-    return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+    return e == hipSuccess ? SVPF_RC_OK : SVPF_RC_HIP;
 }
 
 size_t svpf_diff2d_tape_floats(int n_particles, int n_stein, int checkpoint_every) {
@@ -1157,19 +1160,19 @@ size_t svpf_diff2d_tape_floats(int n_particles, int n_stein, int checkpoint_ever
 }
 
 int svpf_diff2d_forward(const void* config, const void* d_desc, int B, hipStream_t stream) {
-    if (!config || !d_desc || B < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!config || !d_desc || B < 1) return SVPF_RC_INVALID_ARGUMENT;
     const svpf_d2config* c = static_cast<const svpf_d2config*>(config);
-    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_RC_INVALID_ARGUMENT;
     hipError_t e = svpf_launch_diff2d(0, c, static_cast<const svpf_d2desc*>(d_desc), B, stream);
-    return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+    return e == hipSuccess ? SVPF_RC_OK : SVPF_RC_HIP;
 }
 
 int svpf_diff2d_backward(const void* config, const void* d_desc, int B, hipStream_t stream) {
-    if (!config || !d_desc || B < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    if (!config || !d_desc || B < 1) return SVPF_RC_INVALID_ARGUMENT;
     const svpf_d2config* c = static_cast<const svpf_d2config*>(config);
-    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_ERR_INVALID_ARGUMENT;
+    if (c->n < 2 || c->n > 2048 || c->S < 1 || c->T < 1 || c->C < 1) return SVPF_RC_INVALID_ARGUMENT;
     hipError_t e = svpf_launch_diff2d(1, c, static_cast<const svpf_d2desc*>(d_desc), B, stream);
-    return e == hipSuccess ? SVPF_OK : SVPF_ERR_HIP;
+    return e == hipSuccess ? SVPF_RC_OK : SVPF_RC_HIP;
 }
 
 int svpf_tile_capacity_query(int tile, int block_size) { return svpf_tile_capacity(tile, block_size); }
@@ -1178,3 +1181,15 @@ size_t svpf_lds_bytes_query(int nmax) { return svpf_lds_bytes(nmax); }
 const char* svpf_version(void) { return "svpf-mi355x 0.1.0 (gfx950)"; }
 
 }  // extern "C"
+
+float* svpf_internal_diag_dev(const SVPFState* s) {
+    return s && s->impl ? reinterpret_cast<float*>(s->impl->dblock + kBlockDiag) : nullptr;
+}
+float* svpf_internal_diag_host(const SVPFState* s) {
+    return s && s->impl ? reinterpret_cast<float*>(s->impl->hblock + kBlockDiag) : nullptr;
+}
+void svpf_internal_set_error(const SVPFState* s, int rc, hipError_t e) {
+    if (!s || !s->impl) return;
+    s->impl->last_error = rc;
+    if (rc == SVPF_RC_HIP) s->impl->last_hip = e;
+}

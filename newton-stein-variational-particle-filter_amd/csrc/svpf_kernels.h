@@ -90,3 +90,17 @@ hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, const svpf_fd
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+// Diagnostics beside the step (one workgroup, O(n)).  HEALTH: out0[0..6] =
+// NaN count, Inf count, sum w, sum w^2, min h, max h, sum h (finite h only).
+// NU (a = y_t, b = nu) and SIGMA (h = h_pred, h2 = h_prev, a = mu, b = rho,
+// c = sigma): *out0 = weighted mean score, *out1 = weighted mean z^2 or eps^2/sigma^2.
+enum { SVPF_DIAG_HEALTH = 0, SVPF_DIAG_NU = 1, SVPF_DIAG_SIGMA = 2 };
+hipError_t svpf_launch_diag(int mode, const float* h, const float* h2, const float* lw, int n, float a, float b,
+                            float c, float* out0, float* out1, hipStream_t stream);
+#ifdef __cplusplus
+}
+#endif

@@ -69,13 +69,13 @@ class SVPFState(ctypes.Structure):
 
 
 # Status codes (include/svpf.h)
-SVPF_OK = 0
-SVPF_ERR_INVALID_ARGUMENT = 1
-SVPF_ERR_UNSUPPORTED = 2
-SVPF_ERR_HIP = 3
-SVPF_ERR_NOT_INITIALIZED = 4
-SVPF_ERR_OUT_OF_MEMORY = 5
-SVPF_ERR_TIMEOUT = 6
+SVPF_RC_OK = 0
+SVPF_RC_INVALID_ARGUMENT = 1
+SVPF_RC_UNSUPPORTED = 2
+SVPF_RC_HIP = 3
+SVPF_RC_NOT_INITIALIZED = 4
+SVPF_RC_OUT_OF_MEMORY = 5
+SVPF_RC_TIMEOUT = 6
 
 
 
@@ -86,7 +86,8 @@ class SVPFV2Noise(ctypes.Structure):
                 ("z_rejuv", c_void_p)]
 
 
-# Every symbol include/svpf.h and include/svpf_fully_fused_v2.h declare (non-inline).
+# Every symbol include/svpf.h, include/svpf_fully_fused_v2.h and
+# include/svpf_diagnostics.h declare (non-inline).
 EXPORTED = (
     "svpf_create", "svpf_destroy", "svpf_initialize", "svpf_step", "svpf_step_seeded", "svpf_run_sequence",
     "svpf_run_sequence_device", "svpf_step_adaptive", "svpf_step_graph", "svpf_graph_is_captured",
@@ -98,6 +99,15 @@ EXPORTED = (
     "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
     "svpf_set_grid_path", "svpf_get_path", "svpf_fully_fused_step_v2", "svpf_fully_fused_v2_supported",
     "svpf_coop_launch_is_cooperative", "svpf_set_coop_spin_limit",
+    # score diagnostics (include/svpf.h)
+    "svpf_gradient_diagnostic_create", "svpf_gradient_diagnostic_destroy", "svpf_compute_nu_diagnostic",
+    "svpf_compute_nu_diagnostic_simple", "svpf_compute_sigma_diagnostic_simple", "svpf_snapshot_particles",
+    "svpf_update_shock_state", "svpf_get_shock_state", "svpf_should_learn", "svpf_get_lr_multiplier",
+    "svpf_tuner_create", "svpf_tuner_destroy", "svpf_tuner_update", "svpf_tuner_get_params",
+    # health checks (include/svpf_diagnostics.h)
+    "svpf_diag_init", "svpf_diag_check", "svpf_diag_check_input", "svpf_diag_message", "svpf_status_name",
+    "svpf_status_is_error", "svpf_status_is_warning", "svpf_diag_clear_error", "svpf_diag_print_summary",
+    "svpf_diag_to_json",
 )
 
 # svpf_fully_fused_step_v2 argument kinds, in order (include/svpf_fully_fused_v2.h)
@@ -214,5 +224,5 @@ def load(path: str | None = None):
 def check(state_ptr, what: str):
     L = load()
     rc = L.svpf_last_error(state_ptr)
-    if rc != SVPF_OK:
+    if rc != SVPF_RC_OK:
         raise RuntimeError(f"{what} failed: {L.svpf_last_error_string(state_ptr).decode()} (status {rc})")

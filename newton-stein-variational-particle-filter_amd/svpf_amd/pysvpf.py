@@ -174,7 +174,7 @@ class SVPF:
     # ---- core API (py-torch/svpf_python.cpp:320-415) ----
     def _check(self, what):
         rc = self._L.svpf_last_error(self._sp)
-        if rc != _lib.SVPF_OK:
+        if rc != _lib.SVPF_RC_OK:
             raise RuntimeError(f"{what} failed: {self._L.svpf_last_error_string(self._sp).decode()}")
 
     def initialize(self, rho=0.97, sigma_z=0.15, mu=-3.5, gamma=-0.5, seed=42):

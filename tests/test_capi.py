@@ -1,5 +1,5 @@
 """C ABI checks that need no GPU: the library loads, exports every function
-include/svpf.h and include/svpf_fully_fused_v2.h declare, and the ctypes
+include/svpf.h, include/svpf_fully_fused_v2.h and include/svpf_diagnostics.h declare, and the ctypes
 mirrors of SVPFState and svpf_v2_noise match the C layouts field by field."""
 import ctypes
 import os
@@ -12,7 +12,7 @@ from svpf_amd import _lib
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "svpf.h")
-HEADERS = [os.path.join(ROOT, "include", f) for f in ("svpf.h", "svpf_fully_fused_v2.h")]
+HEADERS = [os.path.join(ROOT, "include", f) for f in ("svpf.h", "svpf_fully_fused_v2.h", "svpf_diagnostics.h")]
 
 
 def declared_functions():

@@ -169,17 +169,17 @@ def test_grid_path_switching(torch_cuda):
 
     f = Filter(config.plain_config(n=256))
     assert f.path() == "coop"  # automatic choice at 256 <= n < 1024 with the cooperative kernel's features
-    assert f.L.svpf_set_grid_path(f.sp, 1) == _lib.SVPF_OK and f.path() == "grid"
-    assert f.L.svpf_set_grid_path(f.sp, 0) == _lib.SVPF_OK and f.path() == "coop"
+    assert f.L.svpf_set_grid_path(f.sp, 1) == _lib.SVPF_RC_OK and f.path() == "grid"
+    assert f.L.svpf_set_grid_path(f.sp, 0) == _lib.SVPF_RC_OK and f.path() == "coop"
     f.state.use_persistent_kernel = 0
     assert f.path() == "single"
-    assert f.L.svpf_set_grid_path(f.sp, -1) == _lib.SVPF_OK and f.path() == "grid"  # no cooperative kernel
-    assert f.L.svpf_set_grid_path(f.sp, 2) == _lib.SVPF_ERR_INVALID_ARGUMENT
+    assert f.L.svpf_set_grid_path(f.sp, -1) == _lib.SVPF_RC_OK and f.path() == "grid"  # no cooperative kernel
+    assert f.L.svpf_set_grid_path(f.sp, 2) == _lib.SVPF_RC_INVALID_ARGUMENT
     e = Filter(config.full_feature_config(n=512))  # KSD early stop: no cooperative kernel
     assert e.path() == "grid"
     assert Filter(config.plain_config(n=128)).path() == "single"
     assert Filter(config.plain_config(n=1024)).path() == "grid"
     g = Filter(config.plain_config(n=16384))
     assert g.path() == "grid"
-    assert g.L.svpf_set_grid_path(g.sp, 0) == _lib.SVPF_ERR_UNSUPPORTED
+    assert g.L.svpf_set_grid_path(g.sp, 0) == _lib.SVPF_RC_UNSUPPORTED
     assert not g.L.svpf_create(16385, 5, 5.0, None)

@@ -203,10 +203,10 @@ def test_paths(torch_cuda):
         assert Filter(config.plain_config(n=n, use_heun=1, use_student_t_state=1)).path() == "grid", n
     assert Filter(config.plain_config(n=512, use_student_t_state=1), path="single").path() == "single"
     big = Filter(config.plain_config(n=8192, use_heun=1))
-    assert big.L.svpf_set_grid_path(big.sp, 0) == _lib.SVPF_ERR_UNSUPPORTED
+    assert big.L.svpf_set_grid_path(big.sp, 0) == _lib.SVPF_RC_UNSUPPORTED
     big.initialize(3)
     big.step(0.1, 0.0)
-    assert big.L.svpf_last_error(big.sp) == _lib.SVPF_OK
+    assert big.L.svpf_last_error(big.sp) == _lib.SVPF_RC_OK
 
 
 def test_grid_heun_student_n16384_one_tick(torch_cuda):

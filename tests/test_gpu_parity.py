@@ -202,7 +202,7 @@ def test_unsupported_and_uninitialized_fail_loudly(torch_cuda):
         f.step(0.1, 0.0)
     f.state.use_two_factor = 0
     f.step(0.1, 0.0)
-    assert f.L.svpf_last_error(f.sp) == _lib.SVPF_OK
+    assert f.L.svpf_last_error(f.sp) == _lib.SVPF_RC_OK
     assert not f.L.svpf_create(0, 5, 5.0, None)
 
 
@@ -255,7 +255,7 @@ def test_coop_exchange_timeout_repeats_with_a_cooperative_launch(torch_cuda):
     """The cooperative kernel runs as a plain launch.  A forced time-out (the
     exchanges wait for a tag nobody publishes, 4 polls) writes nothing back; the
     step is repeated with a cooperative launch (forced to time out as well, so it
-    reports SVPF_ERR_TIMEOUT) and the filter keeps cooperative launches.  With
+    reports SVPF_RC_TIMEOUT) and the filter keeps cooperative launches.  With
     the default bound its next tick equals that of a filter that never timed out."""
     from svpf_amd import _lib
 
@@ -275,19 +275,19 @@ def test_coop_exchange_timeout_repeats_with_a_cooperative_launch(torch_cuda):
     f = make()
     L = f.L
     assert L.svpf_coop_launch_is_cooperative(f.sp) == 0
-    assert L.svpf_set_coop_spin_limit(f.sp, -4) == _lib.SVPF_OK
+    assert L.svpf_set_coop_spin_limit(f.sp, -4) == _lib.SVPF_RC_OK
     print("coop exchange time-out", flush=True)
     with pytest.raises(RuntimeError, match="time"):
         f.step(y0, 0.0)
-    assert L.svpf_last_error(f.sp) == _lib.SVPF_ERR_TIMEOUT
+    assert L.svpf_last_error(f.sp) == _lib.SVPF_RC_TIMEOUT
     assert L.svpf_coop_launch_is_cooperative(f.sp) == 1
     assert np.array_equal(f.particles(), h0.astype(np.float32))
     assert f.state.timestep == 0
-    assert L.svpf_set_coop_spin_limit(f.sp, 0) == _lib.SVPF_OK
+    assert L.svpf_set_coop_spin_limit(f.sp, 0) == _lib.SVPF_RC_OK
     f.step(y0, 0.0)
     g = make()
     g.step(y0, 0.0)
     assert L.svpf_coop_launch_is_cooperative(g.sp) == 0
     assert np.array_equal(f.particles(), g.particles())
     assert np.array_equal(f.last_outputs(), g.last_outputs())
-    assert L.svpf_set_coop_spin_limit(f.sp, -(1 << 21)) == _lib.SVPF_ERR_INVALID_ARGUMENT
+    assert L.svpf_set_coop_spin_limit(f.sp, -(1 << 21)) == _lib.SVPF_RC_INVALID_ARGUMENT
