@@ -11,6 +11,8 @@ the oracle, the CPU torch path and the device path consume identical inputs.
 """
 from __future__ import annotations
 
+import re
+
 import numpy as np
 
 
@@ -102,3 +104,44 @@ def batched_thetas(B, seed=0, sizes=(128, 256, 512, 1024, 2048)):
     sz = rng.uniform(0.05, 0.4, B)
     nu = rng.uniform(3.0, 30.0, B)
     return n.astype(np.int32), mu, rho, sz, nu
+
+
+# ---- market-data formats (include/market_data_loader.h of this repository,
+# the reference's include/market_data_loader.h:37-237) ----
+
+def load_returns_binary(path) -> np.ndarray:
+    """<name>.bin: raw little-endian float32 returns."""
+    return np.fromfile(path, dtype="<f4").astype(np.float32)
+
+
+def load_returns_csv(path) -> np.ndarray:
+    """<name>_returns.csv: one return per line (a blank line reads as 0, as atof does)."""
+    with open(path) as f:
+        return np.array([_atof(ln) for ln in f], dtype=np.float32)
+
+
+def load_market_data_csv(path):
+    """<name>.csv: header, then date,price,return,log_return.  Returns (log_returns, prices) float32."""
+    with open(path) as f:
+        rows = f.read().splitlines()[1:]
+    cols = [r.split(",") for r in rows]
+    ret = np.array([_atof(c[3]) if len(c) > 3 else 0.0 for c in cols], dtype=np.float32)
+    px = np.array([_atof(c[1]) if len(c) > 1 else 0.0 for c in cols], dtype=np.float32)
+    return ret, px
+
+
+def return_stats(r):
+    """(mean, std, min, max, n_3sigma, n_4sigma) as compute_return_stats: sums in
+    double, population std, |r - mean| / std tested in float32."""
+    r = np.asarray(r, dtype=np.float32)
+    m = np.float32(np.sum(r.astype(np.float64)) / len(r))
+    d = (r - m).astype(np.float64)
+    sd = np.float32(np.sqrt(np.sum(d * d) / len(r)))
+    z = np.abs((r - m) / sd)
+    return float(m), float(sd), float(r.min()), float(r.max()), int((z > 3.0).sum()), int((z > 4.0).sum())
+
+
+def _atof(s: str) -> float:
+    """C atof: the longest leading decimal number, 0.0 if there is none."""
+    m = re.match(r"\s*([+-]?(\d+\.?\d*|\.\d+)([eE][+-]?\d+)?)", s)
+    return float(m.group(1)) if m else 0.0

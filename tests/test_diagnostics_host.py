@@ -116,3 +116,25 @@ def test_tuner_maps_parameters():
     L.svpf_tuner_update(t, None, ctypes.byref(q), ctypes.byref(nu))
     assert np.allclose([q.rho, q.sigma_z, q.mu, q.gamma, nu.value], [0.97, 0.15, -1.0, 0.0, 7.0], atol=1e-5)
     L.svpf_tuner_destroy(t)
+
+
+def test_multiscale_layouts_exports_and_defaults(tmp_path):
+    from svpf_amd import multiscale as ms
+
+    for cls, name in ((ms.MSConfig, "MS_SVPF_Config"), (ms.MSOutput, "MS_SVPF_Output"), (ms.MSState, "MS_SVPF")):
+        fields = [f for f, _ in cls._fields_]
+        got = _c_layout(tmp_path, "svpf_multiscale.h", name, fields)
+        assert got["size"] == ctypes.sizeof(cls), name
+        for f in fields:
+            assert got[f] == getattr(cls, f).offset, (name, f)
+    L = ms.bind()
+    for name in ms.EXPORTED:
+        assert hasattr(L, name), name
+    c = L.ms_svpf_default_config()
+    assert (c.n_particles_reactive, c.n_particles_inertial, c.n_stein_reactive, c.n_stein_inertial) == (256, 1024, 3, 8)
+    assert np.allclose([c.surprise_mild_threshold, c.surprise_high_threshold, c.boost_mild_factor,
+                        c.boost_high_factor, c.weight_reactive, c.weight_inertial], [2.0, 3.5, 1.5, 2.5, 0.1, 0.9])
+    o = ms.MSOutput(vol_reactive=0.03, vol_inertial=0.01)
+    assert abs(L.ms_svpf_vol_for_execution(ctypes.byref(o)) - (0.7 * 0.03 + 0.3 * 0.01)) < 1e-8
+    assert abs(L.ms_svpf_vol_for_position(ctypes.byref(o)) - (0.2 * 0.03 + 0.8 * 0.01)) < 1e-8
+    assert abs(L.ms_svpf_vol_conservative(ctypes.byref(o)) - 0.03) < 1e-8
