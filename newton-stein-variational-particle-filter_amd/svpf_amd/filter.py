@@ -171,6 +171,18 @@ class Filter:
         self.L.svpf_get_particles(self.sp, h.ctypes.data)
         return h
 
+    def log_weights(self):
+        """The log-weights of the last step (device array state.log_weights), read back synchronously."""
+        lw = np.zeros(self.n, dtype=np.float32)
+        hip = ctypes.CDLL("libamdhip64.so")
+        hip.hipDeviceSynchronize()
+        rc = hip.hipMemcpy(lw.ctypes.data_as(ctypes.c_void_p),
+                           ctypes.c_void_p(ctypes.cast(self.state.log_weights, ctypes.c_void_p).value),
+                           ctypes.c_size_t(4 * self.n), 2)  # device to host
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy of the log-weights failed ({rc})")
+        return lw
+
     def rmsprop_state(self):
         v = np.zeros(self.n, dtype=np.float32)
         self.L.svpf_get_rmsprop_state(self.sp, v.ctypes.data)
