@@ -9,9 +9,10 @@
  * Differences from the reference, all additive:
  *   - errors are reported: every call records a status readable through
  *     svpf_last_error(state); svpf_create returns NULL on any failure;
- *   - features the reference offers but this build does not implement
- *     (Student-t state noise, Heun integration, the two-factor branch) are
- *     rejected with SVPF_RC_UNSUPPORTED instead of being silently ignored;
+ *   - a feature combination a kernel path cannot serve is rejected with
+ *     SVPF_RC_UNSUPPORTED instead of being silently ignored (the
+ *     coordinate-wise two-factor step, use_two_factor, runs in one workgroup
+ *     and so takes n_particles up to the single-workgroup capacity);
  *   - svpf_step_graph really replays a captured HIP graph per tick;
  *   - three configuration fields: bandwidth_mode, use_linear_beta,
  *     use_ksd_early_stop (defaults reproduce the reference).
@@ -168,7 +169,7 @@ typedef struct {
     float ksd_prev;
     int stein_steps_used;
 
-    int use_student_t_state; /* single-workgroup kernel only (n <= its capacity); else SVPF_RC_UNSUPPORTED */
+    int use_student_t_state; /* single-workgroup kernel (n < 256) or grid path; not the cooperative kernel */
     float nu_state;
 
     int use_smoothing;
@@ -181,8 +182,25 @@ typedef struct {
 
     int use_persistent_kernel; /* 1: cooperative multi-workgroup kernel (default for n >= 256) */
     int persistent_kernel_supported;
-    int use_heun;      /* single-workgroup kernel only (n <= its capacity); else SVPF_RC_UNSUPPORTED */
-    int use_two_factor; /* rejected: SVPF_RC_UNSUPPORTED */
+    int use_heun; /* single-workgroup kernel (n < 256) or grid path; not the cooperative kernel */
+
+    /* include/svpf.cuh:533-544: the coordinate-wise two-factor step
+     * (src/svpf_optimized_graph.cu:744-897), h = mu + h_fast + h_slow.  One
+     * workgroup, n_particles up to the single-workgroup capacity
+     * (svpf_tile_capacity_query); above it the step reports
+     * SVPF_RC_UNSUPPORTED.  guide_* are read and written by each tick. */
+    int use_two_factor;
+    float rho_fast;
+    float sigma_fast;
+    float rho_slow;
+    float sigma_slow;
+    float bw_floor_fast;
+    float bw_floor_slow;
+    float guide_mean_fast;
+    float guide_mean_slow;
+    float guide_var_fast;
+    float guide_var_slow;
+    int guide_2f_initialized;
 
     /* extensions (defaults reproduce the reference) */
     int bandwidth_mode;
@@ -379,6 +397,10 @@ int svpf_tile_occupancy_query(int tile, int block_size, int nmax);
 size_t svpf_lds_bytes_query(int nmax);
 /* Library version string. */
 const char* svpf_version(void);
+/* No counterpart in the reference: the bounded digamma the adjoint and score
+ * kernels use (the same code, compiled for the host), for host-side checks.
+ * NaN for a non-positive or non-finite argument. */
+float svpf_digamma_f32(float x);
 
 /* ---- Score diagnostics beside the step (include/svpf.cuh:895-1213) ----
  * Posterior-weighted scores of the SV model read from the filter after a

@@ -23,6 +23,7 @@
 #include "../../include/svpf.h"
 #include "svpf_internal.h"
 #include "svpf_kernels.h"
+#include "svpf_special.h"
 #include "svpf_types.h"
 
 namespace {
@@ -1178,7 +1179,8 @@ int svpf_diff2d_backward(const void* config, const void* d_desc, int B, hipStrea
 int svpf_tile_capacity_query(int tile, int block_size) { return svpf_tile_capacity(tile, block_size); }
 int svpf_tile_occupancy_query(int tile, int block_size, int nmax) { return svpf_batched_wg_per_cu(block_size, tile, nmax); }
 size_t svpf_lds_bytes_query(int nmax) { return svpf_lds_bytes(nmax); }
-const char* svpf_version(void) { return "svpf-mi355x 0.1.0 (gfx950)"; }
+const char* svpf_version(void) { return "svpf-mi355x 0.2.0 (gfx950)"; }
+float svpf_digamma_f32(float x) { return svpf_digamma(x); }
 
 }  // extern "C"
 

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include "svpf_kernels.h"
+#include "svpf_special.h"
 
 namespace {
 
@@ -48,24 +49,6 @@ __device__ float block_reduce(float x, float* red, Op op) {
 #pragma unroll
     for (int w = 1; w < kWaves; ++w) r = op(r, red[w]);
     return r;
-}
-
-// psi(x) for x > 0: the recurrence psi(x) = psi(x + 1) - 1/x lifts x to >= 6
-// (at most 6 steps for any x > 0), then the asymptotic series
-// ln x - 1/(2x) - 1/(12x^2) + 1/(120x^4) - 1/(252x^6) (Abramowitz & Stegun
-// 6.3.18), whose next term is below 3e-9 there.
-__device__ float digamma(float x) {
-    float r = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        if (x < 6.0f) {
-            r -= 1.0f / x;
-            x += 1.0f;
-        }
-    }
-    const float ix = 1.0f / x, ix2 = ix * ix;
-    const float series = ix2 * (-1.0f / 12.0f + ix2 * (1.0f / 120.0f - ix2 * (1.0f / 252.0f)));
-    return r + logf(x) - 0.5f * ix + series;
 }
 
 __device__ bool finite_h(float h) { return h == h && h > -1e10f && h < 1e10f; }
@@ -130,7 +113,7 @@ __global__ __launch_bounds__(kDiagBS) void svpf_diag_kernel(int mode, const floa
     if (mode == SVPF_DIAG_NU) {
         // a = y_t, b = nu
         const float nu = b, inv_nu = 1.0f / nu, nu1 = nu + 1.0f;
-        const float psi_d = digamma(0.5f * nu1) - digamma(0.5f * nu);
+        const float psi_d = svpf_digamma(0.5f * nu1) - svpf_digamma(0.5f * nu);
         for (int i = tid; i < n; i += kDiagBS) {
             const float w = expf(lw[i] - mx);
             const float z = a / (expf(0.5f * h[i]) + 1e-8f);

@@ -37,6 +37,7 @@
 #pragma once
 
 #include "svpf_device.hpp"
+#include "svpf_special.h"
 
 namespace svpf {
 
@@ -48,19 +49,6 @@ constexpr float kDiffAlpha = 0.8493218002880191f;  // sqrt(log2 e / 2)
 // then per iteration the TP_ROWS rows of n floats, then 4 scalars.
 enum { TP_H = 0, TP_V = 1, TP_PHI = 2, TP_EDD = 3, TP_VDD = 4, TP_CDD = 5, TP_L = 6, TP_UN = 7, TP_ROWS = 8 };
 __host__ __device__ inline size_t diff_slot_floats(int n, int S) { return (size_t)(2 + TP_ROWS * S) * n + 4; }
-
-// digamma for x > 0: recurrence up to x >= 6, then the asymptotic series
-// ln x - 1/2x - sum B_2k / (2k x^2k) (Abramowitz & Stegun 6.3.18).
-SVPF_DEV float digamma_pos(float x) {
-    float r = 0.0f;
-    while (x < 6.0f) {
-        r -= 1.0f / x;
-        x += 1.0f;
-    }
-    const float f = 1.0f / (x * x);
-    const float tail = f * (1.0f / 12.0f - f * (1.0f / 120.0f - f * (1.0f / 252.0f - f * (1.0f / 240.0f - f / 132.0f))));
-    return r + logf(x) - 0.5f / x - tail;
-}
 
 struct DiffTheta {
     float mu, rho, sigma, nu;
@@ -75,7 +63,7 @@ SVPF_DEV DiffTheta diff_theta(const float* th) {
     t.sigma = th[2];
     t.nu = th[3];
     t.st_const = lgammaf((t.nu + 1.0f) * 0.5f) - lgammaf(t.nu * 0.5f) - 0.5f * logf(3.14159265358979f * t.nu);
-    t.dst_const = 0.5f * digamma_pos((t.nu + 1.0f) * 0.5f) - 0.5f * digamma_pos(t.nu * 0.5f) - 0.5f / t.nu;
+    t.dst_const = 0.5f * svpf_digamma((t.nu + 1.0f) * 0.5f) - 0.5f * svpf_digamma(t.nu * 0.5f) - 0.5f / t.nu;
     return t;
 }
 

@@ -51,7 +51,7 @@ SVPF_DEV D2Theta d2_theta(const float* th) {
     t.sig_s = th[4];
     t.nu = th[5];
     t.st_const = lgammaf((t.nu + 1.0f) * 0.5f) - lgammaf(t.nu * 0.5f) - 0.5f * logf(3.14159265358979f * t.nu);
-    t.dst_const = 0.5f * digamma_pos((t.nu + 1.0f) * 0.5f) - 0.5f * digamma_pos(t.nu * 0.5f) - 0.5f / t.nu;
+    t.dst_const = 0.5f * svpf_digamma((t.nu + 1.0f) * 0.5f) - 0.5f * svpf_digamma(t.nu * 0.5f) - 0.5f / t.nu;
     return t;
 }
 

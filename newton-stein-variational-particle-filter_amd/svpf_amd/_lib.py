@@ -58,7 +58,10 @@ _STATE_FIELDS = [
     ("smooth_h_mean", c_float * SMOOTH_MAX_LAG), ("smooth_h_var", c_float * SMOOTH_MAX_LAG),
     ("smooth_y", c_float * SMOOTH_MAX_LAG),
     ("use_persistent_kernel", c_int), ("persistent_kernel_supported", c_int), ("use_heun", c_int),
-    ("use_two_factor", c_int),
+    ("use_two_factor", c_int), ("rho_fast", c_float), ("sigma_fast", c_float), ("rho_slow", c_float),
+    ("sigma_slow", c_float), ("bw_floor_fast", c_float), ("bw_floor_slow", c_float),
+    ("guide_mean_fast", c_float), ("guide_mean_slow", c_float), ("guide_var_fast", c_float),
+    ("guide_var_slow", c_float), ("guide_2f_initialized", c_int),
     ("bandwidth_mode", c_int), ("use_linear_beta", c_int), ("use_ksd_early_stop", c_int),
     ("impl", c_void_p),
 ]
@@ -98,7 +101,7 @@ EXPORTED = (
     "svpf_last_error_string", "svpf_diff_tape_floats", "svpf_diff_forward", "svpf_diff_backward",
     "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
     "svpf_set_grid_path", "svpf_get_path", "svpf_fully_fused_step_v2", "svpf_fully_fused_v2_supported",
-    "svpf_coop_launch_is_cooperative", "svpf_set_coop_spin_limit",
+    "svpf_coop_launch_is_cooperative", "svpf_set_coop_spin_limit", "svpf_digamma_f32",
     # score diagnostics (include/svpf.h)
     "svpf_gradient_diagnostic_create", "svpf_gradient_diagnostic_destroy", "svpf_compute_nu_diagnostic",
     "svpf_compute_nu_diagnostic_simple", "svpf_compute_sigma_diagnostic_simple", "svpf_snapshot_particles",
@@ -216,6 +219,8 @@ def load(path: str | None = None):
     L.svpf_fully_fused_v2_supported.argtypes = [c_int]
     L.svpf_version.restype = ctypes.c_char_p
     L.svpf_version.argtypes = []
+    L.svpf_digamma_f32.restype = c_float
+    L.svpf_digamma_f32.argtypes = [c_float]
     if path is None:
         _lib = L
     return L

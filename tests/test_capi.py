@@ -142,3 +142,18 @@ def test_v2_noise_layout_matches_header(tmp_path):
     L = _lib.load()
     assert L.svpf_fully_fused_v2_supported(1024) and not L.svpf_fully_fused_v2_supported(1025)
     assert not L.svpf_fully_fused_v2_supported(0)
+
+
+def test_digamma_is_bounded_and_accurate():
+    """The adjoint and score kernels' digamma (compiled for the host from the same
+    source): accurate for x > 0, NaN -- not an endless recurrence -- for a
+    non-positive or non-finite argument such as a diverged degrees of freedom."""
+    import math
+
+    import scipy.special
+
+    L = _lib.load()
+    for x in (1e-3, 0.5, 1.0, 2.5, 5.0, 6.0, 17.3, 1e4):
+        assert L.svpf_digamma_f32(x) == pytest.approx(scipy.special.digamma(x), rel=3e-6, abs=3e-6), x
+    for x in (0.0, -1.0, -0.5, -3e7, float("-inf"), float("inf"), float("nan")):
+        assert math.isnan(L.svpf_digamma_f32(x)), x
