@@ -333,3 +333,116 @@ def v2_step(args: dict, h, h_prev, v, io: dict, noise=None, seed=0, filter_id=0)
     if rc:
         raise RuntimeError("oracle v2 step failed")
     return grad, log_w, inv_hess, {f: getattr(st, f) for f, _ in V2IO._fields_}
+
+
+# ---------------------------------------------------------------- initialisation, smoothing, two-factor
+def init_particles(n, mu, rho, sigma_z, seed, student_t_state=0, nu_state=5.0):
+    """svpfo_init_particles: svpf_initialize's particle draw (float64)."""
+    L = lib()
+    L.svpfo_init_particles.restype = None
+    L.svpfo_init_particles.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_uint64, ctypes.c_void_p]
+    h = np.zeros(n)
+    L.svpfo_init_particles(n, mu, rho, sigma_z, int(student_t_state), nu_state, seed, _ptr(h))
+    return h
+
+
+class Smooth(ctypes.Structure):
+    _fields_ = [("lag", ctypes.c_int), ("output_lag", ctypes.c_int), ("head", ctypes.c_int), ("pad_", ctypes.c_int),
+                ("h_mean", ctypes.c_double * 8), ("h_var", ctypes.c_double * 8), ("y", ctypes.c_double * 8)]
+
+
+class Smoother:
+    """svpfo_smooth_step: the RTS-style correction of the reported h_mean over a
+    window of `lag` ticks (buffers start as svpf_create leaves them)."""
+
+    def __init__(self, lag=3, output_lag=1):
+        self.s = Smooth()
+        self.s.lag, self.s.output_lag, self.s.head = lag, output_lag, 0
+        for i in range(8):
+            self.s.h_var[i] = 1.0
+        L = lib()
+        L.svpfo_smooth_step.restype = ctypes.c_double
+        L.svpfo_smooth_step.argtypes = [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_double] * 6
+
+    def step(self, timestep, h_mean, h_var, y_t, mu, rho, sigma_z):
+        return lib().svpfo_smooth_step(ctypes.byref(self.s), int(timestep), h_mean, h_var, y_t, mu, rho, sigma_z)
+
+
+TWOF_INT_FIELDS = ("n", "n_stein_steps", "n_anneal_steps", "use_annealing", "use_svld", "use_mim", "use_guide",
+                   "use_guide_preserving", "use_exact_gradient", "stein_sign", "pad0_", "pad1_")
+TWOF_DOUBLE_FIELDS = ("mu", "nu", "rho_fast", "sigma_fast", "rho_slow", "sigma_slow", "bw_floor_fast",
+                      "bw_floor_slow", "guide_strength", "mim_jump_prob", "mim_jump_scale", "temperature",
+                      "rmsprop_rho", "rmsprop_eps")
+
+
+class Cfg2F(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int) for f in TWOF_INT_FIELDS] + [(f, ctypes.c_double) for f in TWOF_DOUBLE_FIELDS]
+
+
+class State2F(ctypes.Structure):
+    _fields_ = [("timestep", ctypes.c_int), ("guide_2f_initialized", ctypes.c_int)] + [
+        (f, ctypes.c_double) for f in ("guide_mean_fast", "guide_mean_slow", "guide_var_fast", "guide_var_slow",
+                                       "mean_fast_prev", "mean_slow_prev", "vol_prev", "h_mean_prev")]
+
+
+class Out2F(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_double) for f in ("loglik", "pred_loglik", "vol", "h_mean", "mean_fast", "mean_slow",
+                                               "bw_fast", "bw_slow")] + [("steps", ctypes.c_int),
+                                                                         ("pad_", ctypes.c_int)]
+
+
+def _lib2f():
+    L = lib()
+    P = ctypes.c_void_p
+    L.svpfo_tick_2f.restype = ctypes.c_int
+    L.svpfo_tick_2f.argtypes = [P, P, P, P, P, P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32, P]
+    L.svpfo_ekf_2f_update.restype = None
+    L.svpfo_ekf_2f_update.argtypes = [P, P, ctypes.c_double]
+    L.svpfo_init_2f.restype = None
+    L.svpfo_init_2f.argtypes = [ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_uint64, P, P]
+    return L
+
+
+def make_cfg_2f(cfg: dict) -> Cfg2F:
+    c = Cfg2F()
+    for f in TWOF_INT_FIELDS:
+        if not f.startswith("pad"):
+            setattr(c, f, int(cfg[f]))
+    for f in TWOF_DOUBLE_FIELDS:
+        setattr(c, f, float(cfg[f]))
+    return c
+
+
+def init_2f(n, rho_fast, sigma_fast, rho_slow, sigma_slow, seed):
+    hf, hs = np.zeros(n), np.zeros(n)
+    _lib2f().svpfo_init_2f(n, rho_fast, sigma_fast, rho_slow, sigma_slow, seed, _ptr(hf), _ptr(hs))
+    return hf, hs
+
+
+class TwoFactorOracle:
+    """The coordinate-wise two-factor step run by the oracle (float64 arrays)."""
+
+    def __init__(self, cfg: dict, hf0, hs0, hfp0=None, hsp0=None, vf0=None, vs0=None, **state_kw):
+        n = int(cfg["n"])
+        self.cfg = dict(cfg)
+        self.c = make_cfg_2f(cfg)
+        self.st = State2F()
+        for k, v in state_kw.items():
+            setattr(self.st, k, v)
+        arr = lambda a, d: (np.array(d, dtype=np.float64) if a is None else np.array(a, dtype=np.float64)).reshape(n).copy()
+        self.hf, self.hs = arr(hf0, hf0), arr(hs0, hs0)
+        self.hfp, self.hsp = arr(hfp0, hf0), arr(hsp0, hs0)
+        self.vf, self.vs = arr(vf0, np.zeros(n)), arr(vs0, np.zeros(n))
+
+    def tick(self, y_t, seed=0, filter_id=0):
+        o = Out2F()
+        rc = _lib2f().svpfo_tick_2f(ctypes.byref(self.c), ctypes.byref(self.st), _ptr(self.hf), _ptr(self.hs),
+                                    _ptr(self.hfp), _ptr(self.hsp), _ptr(self.vf), _ptr(self.vs), float(y_t), seed,
+                                    filter_id, ctypes.byref(o))
+        if rc:
+            raise RuntimeError("oracle two-factor tick failed")
+        return {f: getattr(o, f) for f, _ in Out2F._fields_ if f != "pad_"}
+
+    def ekf(self, y_t):
+        _lib2f().svpfo_ekf_2f_update(ctypes.byref(self.c), ctypes.byref(self.st), float(y_t))

@@ -69,7 +69,8 @@ double svpfo_normal_from_bits(uint32_t a, uint32_t b) {
 }
 
 /* Noise purposes (third counter word). */
-enum { PURPOSE_PREDICT = 0, PURPOSE_REJUV = 1, PURPOSE_INIT = 255, PURPOSE_SVLD0 = 16, PURPOSE_STATE_T0 = 128 };
+enum { PURPOSE_PREDICT = 0, PURPOSE_REJUV = 1, PURPOSE_PREDICT_SLOW = 2, PURPOSE_INIT = 255, PURPOSE_INIT_2F = 254,
+       PURPOSE_SVLD0 = 16, PURPOSE_SVLD_SLOW0 = 1024, PURPOSE_STATE_T0 = 128 };
 
 static void philox_draw(uint64_t seed, uint32_t filter, uint32_t tick, uint32_t purpose, uint32_t i,
                         uint32_t out[4]) {
@@ -945,5 +946,246 @@ int svpfo_v2_step(const svpfo_v2_args* a, double* h, double* h_prev, double* gra
     free(hs);
     free(hn);
     free(kp);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* svpf_initialize's particle draw                                          */
+/* ------------------------------------------------------------------------ */
+/* src/svpf_optimized_graph.cu:73-97 (one normal per particle, clamped) and
+ * :349-451 (stationary variance sigma^2/(1-rho^2+1e-6), times nu_s/(nu_s-2)
+ * for the Student-t state with nu_s clamped to >= 2.5). */
+void svpfo_init_particles(int n, double mu, double rho, double sigma_z, int student_t_state, double nu_state,
+                          uint64_t seed, double* h) {
+    double var = sigma_z * sigma_z / (1.0 - rho * rho + 1e-6);
+    if (student_t_state) {
+        double nus = nu_state > 2.5 ? nu_state : 2.5;
+        var *= nus / (nus - 2.0);
+    }
+    double sd = sqrt(var);
+    for (int i = 0; i < n; ++i) {
+        uint32_t o[4];
+        philox_draw(seed, 0u, 0u, PURPOSE_INIT, (uint32_t)i, o);
+        h[i] = clamp_logvol(mu + sd * svpfo_normal_from_bits(o[0], o[1]));
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Smoothing of the reported h_mean (src/svpf_optimized_graph.cu:619-717)    */
+/* ------------------------------------------------------------------------ */
+double svpfo_smooth_step(svpfo_smooth* s, int timestep, double h_mean, double h_var, double y_t, double mu,
+                         double rho, double sigma_z) {
+    int k = s->lag;
+    if (k > 8) k = 8;
+    if (k < 1) k = 1;
+    /* :643-653: store, advance the circular head */
+    int head = s->head;
+    s->h_mean[head] = h_mean;
+    s->h_var[head] = h_var;
+    s->y[head] = y_t;
+    s->head = (head + 1) % k;
+    if (timestep < k) return h_mean;
+    /* :660-690: newest to oldest, each entry corrected from its (corrected) successor */
+    double q = sigma_z * sigma_z;
+    for (int lag = 1; lag < k; ++lag) {
+        int ic = (s->head - lag - 1 + k) % k;
+        int in = (s->head - lag + k) % k;
+        double hc = s->h_mean[ic], hn = s->h_mean[in], vc = s->h_var[ic];
+        double hp = mu + rho * (hc - mu);
+        double pv = rho * rho * vc + q;
+        double J = rho * vc / (pv + 1e-8);
+        s->h_mean[ic] = hc + J * (hn - hp);
+        s->h_var[ic] = vc * (1.0 - J * rho);
+    }
+    /* :701-717: the entry output_lag ticks back */
+    int ol = s->output_lag;
+    if (ol <= 0) return h_mean;
+    if (ol >= k) ol = k - 1;
+    return s->h_mean[(s->head - ol - 1 + k) % k];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Two-factor coordinate-wise step                                          */
+/* ------------------------------------------------------------------------ */
+/* include/svpf_two_factor.cuh:245-300: the factors' means and variances under
+ * AR(1) dynamics; one scalar observation log y^2 = mu + f + s - offset + noise
+ * (variance 4.93 + 2) whose innovation is split by the predicted variances. */
+void svpfo_ekf_2f_update(const svpfo_2f_cfg* c, svpfo_2f_state* s, double y_t) {
+    if (!s->guide_2f_initialized) {
+        s->guide_mean_fast = 0.0;
+        s->guide_mean_slow = 0.0;
+        s->guide_var_fast = c->sigma_fast * c->sigma_fast / (1.0 - c->rho_fast * c->rho_fast);
+        s->guide_var_slow = c->sigma_slow * c->sigma_slow / (1.0 - c->rho_slow * c->rho_slow);
+        s->guide_2f_initialized = 1;
+    }
+    double mf = c->rho_fast * s->guide_mean_fast, ms = c->rho_slow * s->guide_mean_slow;
+    double Pf = c->rho_fast * c->rho_fast * s->guide_var_fast + c->sigma_fast * c->sigma_fast;
+    double Ps = c->rho_slow * c->rho_slow * s->guide_var_slow + c->sigma_slow * c->sigma_slow;
+    double innov = log(y_t * y_t + 1e-8) + svpfo_implied_offset(c->nu) - (c->mu + mf + ms);
+    double S = Pf + Ps + (4.93 + 2.0);
+    double Kf = Pf / (S + 1e-8), Ks = Ps / (S + 1e-8);
+    s->guide_mean_fast = mf + Kf * innov;
+    s->guide_mean_slow = ms + Ks * innov;
+    double vf = (1.0 - Kf) * Pf, vs = (1.0 - Ks) * Ps;
+    s->guide_var_fast = vf > 1e-6 ? vf : 1e-6;
+    s->guide_var_slow = vs > 1e-6 ? vs : 1e-6;
+}
+
+/* src/svpf_two_factor.cu:29-48 */
+void svpfo_init_2f(int n, double rho_fast, double sigma_fast, double rho_slow, double sigma_slow, uint64_t seed,
+                   double* hf, double* hs) {
+    double sf = sigma_fast / sqrt(1.0 - rho_fast * rho_fast + 1e-8);
+    double ss = sigma_slow / sqrt(1.0 - rho_slow * rho_slow + 1e-8);
+    for (int i = 0; i < n; ++i) {
+        uint32_t o[4];
+        philox_draw(seed, 0u, 0u, PURPOSE_INIT_2F, (uint32_t)i, o);
+        hf[i] = clamp_logvol(sf * svpfo_normal_from_bits(o[0], o[1]));
+        hs[i] = clamp_logvol(ss * svpfo_normal_from_bits(o[2], o[3]));
+    }
+}
+
+/* Student-t log density of y at log-variance h, as the two-factor gradient
+ * kernel writes it (src/svpf_two_factor.cu:188-196): no clamp on exp(h). */
+static double log_w_2f(double h, double y, double nu, double stc) {
+    double ratio = y * y / (nu * exp(h) + 1e-8);
+    return stc - 0.5 * h - 0.5 * (nu + 1.0) * log(1.0 + ratio);
+}
+
+/* src/svpf_opt_kernels.cu:527-599 on one coordinate: plain IMQ operator with the
+ * reference sign, RMSProp, step eff, Langevin noise (Jacobi: all from x). */
+static void transport_1d(int n, const double* x, const double* g, double* v, double bw, double eff, double temp,
+                         double rms_rho, double rms_eps, int sign, uint64_t seed, uint32_t filter, uint32_t tick,
+                         uint32_t purpose, double* xn) {
+    double ib = 1.0 / (bw * bw), sgn = sign == 1 ? 1.0 : -1.0;
+    for (int i = 0; i < n; ++i) {
+        double ks = 0.0, gk = 0.0;
+        for (int j = 0; j < n; ++j) {
+            double d = x[i] - x[j], K = 1.0 / (1.0 + d * d * ib);
+            ks += K * g[j];
+            gk += sgn * 2.0 * d * ib * K * K;
+        }
+        double phi = (ks + gk) / n;
+        double vn = rms_rho * v[i] + (1.0 - rms_rho) * phi * phi;
+        v[i] = vn;
+        double diff = 0.0;
+        if (temp > 1e-6) {
+            uint32_t o[4];
+            philox_draw(seed, filter, tick, purpose, (uint32_t)i, o);
+            diff = sqrt(2.0 * eff * temp) * svpfo_normal_from_bits(o[0], o[1]);
+        }
+        xn[i] = clamp_logvol(x[i] + eff * phi / sqrt(vn + rms_eps) + diff);
+    }
+}
+
+/* Silverman bandwidth with a floor (src/svpf_two_factor.cu:256-313): one-pass variance. */
+static double bw_2f(int n, const double* x, double floor_) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = 0; i < n; ++i) { s1 += x[i]; s2 += x[i] * x[i]; }
+    double m = s1 / n, var = s2 / n - m * m;
+    double bw = 1.06 * sqrt(var > 1e-8 ? var : 1e-8) * pow((double)n, -0.2);
+    return bw > floor_ ? bw : floor_;
+}
+
+int svpfo_tick_2f(const svpfo_2f_cfg* c, svpfo_2f_state* st, double* hf, double* hs, double* hfp, double* hsp,
+                  double* vf, double* vs, double y_t, uint64_t seed, uint32_t filter_id, svpfo_2f_out* out) {
+    const int n = c->n;
+    const uint32_t tick = (uint32_t)st->timestep;
+    if (n < 1 || c->n_stein_steps < 1) return 1;
+    double* lw = (double*)calloc((size_t)n, sizeof(double));
+    double* gf = (double*)calloc((size_t)n, sizeof(double));
+    double* gs = (double*)calloc((size_t)n, sizeof(double));
+    double* xn = (double*)calloc((size_t)n, sizeof(double));
+    if (!lw || !gf || !gs || !xn) return -1;
+    const double stc = svpfo_student_t_const(c->nu);
+
+    /* predict (src/svpf_two_factor.cu:55-93): AR(1) of each mean-zero factor, MIM on the fast one */
+    for (int i = 0; i < n; ++i) {
+        uint32_t o[4];
+        philox_draw(seed, filter_id, tick, PURPOSE_PREDICT, (uint32_t)i, o);
+        double zf = svpfo_normal_from_bits(o[0], o[1]);
+        double u = svpfo_u01_from_bits(o[2]);
+        philox_draw(seed, filter_id, tick, PURPOSE_PREDICT_SLOW, (uint32_t)i, o);
+        double zs = svpfo_normal_from_bits(o[0], o[1]);
+        if (c->use_mim && u < c->mim_jump_prob) zf *= c->mim_jump_scale;
+        hfp[i] = hf[i];
+        hsp[i] = hs[i];
+        hf[i] = clamp_logvol(c->rho_fast * hfp[i] + c->sigma_fast * zf);
+        hs[i] = clamp_logvol(c->rho_slow * hsp[i] + c->sigma_slow * zs);
+    }
+    /* predictive log-likelihood of the predicted particles (before the guide) */
+    for (int i = 0; i < n; ++i) lw[i] = log_w_2f(c->mu + hf[i] + hs[i], y_t, c->nu, stc);
+    out->pred_loglik = lse_mean(lw, n);
+
+    /* guide (src/svpf_optimized_graph.cu:780-807, src/svpf_two_factor.cu:320-360): the
+     * factors' means are the previous tick's */
+    if (c->use_guide) {
+        svpfo_ekf_2f_update(c, st, y_t);
+        double g = c->guide_strength, cf = st->mean_fast_prev, cs = st->mean_slow_prev;
+        for (int i = 0; i < n; ++i) {
+            if (c->use_guide_preserving) {
+                hf[i] = clamp_logvol((1.0 - g) * cf + g * st->guide_mean_fast + (hf[i] - cf));
+                hs[i] = clamp_logvol((1.0 - g) * cs + g * st->guide_mean_slow + (hs[i] - cs));
+            } else {
+                hf[i] = clamp_logvol((1.0 - g) * hf[i] + g * st->guide_mean_fast);
+                hs[i] = clamp_logvol((1.0 - g) * hs[i] + g * st->guide_mean_slow);
+            }
+        }
+    }
+
+    /* bandwidths (src/svpf_two_factor.cu:256-313) */
+    double bwf = bw_2f(n, hf, c->bw_floor_fast), bws = bw_2f(n, hs, c->bw_floor_slow);
+
+    /* Stein loop (src/svpf_optimized_graph.cu:822-868): beta = (a+1)/A, step 0.5/S x beta,
+     * gradient then the 1-D transport of each factor */
+    const int na = c->use_annealing ? c->n_anneal_steps : 1;
+    const double base_step = 0.5 / (double)c->n_stein_steps;
+    const double temp = c->use_svld ? c->temperature : 0.0;
+    const double s2f = c->sigma_fast * c->sigma_fast + 1e-8, s2s = c->sigma_slow * c->sigma_slow + 1e-8;
+    const double wf = s2f / (s2f + s2s), ws = 1.0 - wf;
+    int it = 0;
+    for (int a = 0; a < na; ++a) {
+        const double beta = (double)(a + 1) / (double)na;
+        for (int k = 0; k < c->n_stein_steps; ++k, ++it) {
+            /* src/svpf_two_factor.cu:171-249 */
+            for (int i = 0; i < n; ++i) {
+                double h = c->mu + hf[i] + hs[i];
+                double ratio = y_t * y_t / (c->nu * exp(h) + 1e-8);
+                lw[i] = log_w_2f(h, y_t, c->nu, stc);
+                double gl = c->use_exact_gradient ? -0.5 + 0.5 * (c->nu + 1.0) * ratio / (1.0 + ratio + 1e-8)
+                                                  : 0.5 * (log(y_t * y_t + 1e-8) - h);
+                gf[i] = -(hf[i] - c->rho_fast * hfp[i]) / s2f + beta * wf * gl;
+                gs[i] = -(hs[i] - c->rho_slow * hsp[i]) / s2s + beta * ws * gl;
+            }
+            const double eff = base_step * beta;
+            transport_1d(n, hf, gf, vf, bwf, eff, temp, c->rmsprop_rho, c->rmsprop_eps, c->stein_sign, seed,
+                         filter_id, tick, PURPOSE_SVLD0 + (uint32_t)it, xn);
+            memcpy(hf, xn, (size_t)n * sizeof(double));
+            transport_1d(n, hs, gs, vs, bws, eff, temp, c->rmsprop_rho, c->rmsprop_eps, c->stein_sign, seed,
+                         filter_id, tick, PURPOSE_SVLD_SLOW0 + (uint32_t)it, xn);
+            memcpy(hs, xn, (size_t)n * sizeof(double));
+        }
+    }
+
+    /* outputs (src/svpf_two_factor.cu:394-468) */
+    double sf = 0.0, ss = 0.0, sv = 0.0;
+    for (int i = 0; i < n; ++i) {
+        sf += hf[i];
+        ss += hs[i];
+        sv += exp(0.5 * (c->mu + hf[i] + hs[i]));
+    }
+    out->loglik = lse_mean(lw, n);
+    out->mean_fast = sf / n;
+    out->mean_slow = ss / n;
+    out->h_mean = c->mu + out->mean_fast + out->mean_slow;
+    out->vol = sv / n;
+    out->bw_fast = bwf;
+    out->bw_slow = bws;
+    out->steps = it;
+    st->mean_fast_prev = out->mean_fast;
+    st->mean_slow_prev = out->mean_slow;
+    st->vol_prev = out->vol;
+    st->h_mean_prev = out->h_mean;
+    st->timestep++;
+    free(lw); free(gf); free(gs); free(xn);
     return 0;
 }

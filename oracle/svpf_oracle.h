@@ -185,6 +185,58 @@ typedef struct svpfo_v2_io {
 int svpfo_v2_step(const svpfo_v2_args* a, double* h, double* h_prev, double* grad, double* log_w, double* v,
                   double* inv_hess, const svpfo_noise* nz, svpfo_v2_io* io);
 
+/* ---- svpf_initialize's particle draw (src/svpf_optimized_graph.cu:73-97, 349-451) ----
+ * h_i = clamp(mu + sd z_i, -15, 5), z_i the cosine normal of the counter generator
+ * at (particle i, tick 0, purpose 255, filter 0) keyed by seed; sd the stationary
+ * standard deviation sqrt(sigma^2 / (1 - rho^2 + 1e-6)), scaled by
+ * sqrt(nu_s / (nu_s - 2)) (nu_s clamped to >= 2.5) for the Student-t state. */
+void svpfo_init_particles(int n, double mu, double rho, double sigma_z, int student_t_state, double nu_state,
+                          uint64_t seed, double* h);
+
+/* ---- RTS-style smoothing of the reported h_mean (src/svpf_optimized_graph.cu:619-717) ---- */
+typedef struct svpfo_smooth {
+    int lag, output_lag, head, pad_;
+    double h_mean[8], h_var[8], y[8];
+} svpfo_smooth;
+/* Stores (h_mean, h_var, y) of the tick whose pre-increment timestep is `timestep`,
+ * then corrects the window backwards once it is full; returns the output h_mean. */
+double svpfo_smooth_step(svpfo_smooth* s, int timestep, double h_mean, double h_var, double y_t, double mu,
+                         double rho, double sigma_z);
+
+/* ---- the coordinate-wise two-factor step (src/svpf_optimized_graph.cu:744-897,
+ * src/svpf_two_factor.cu:29-468, include/svpf_two_factor.cuh:245-300) ---- */
+typedef struct svpfo_2f_cfg {
+    int n, n_stein_steps, n_anneal_steps, use_annealing;
+    int use_svld, use_mim, use_guide, use_guide_preserving;
+    int use_exact_gradient, stein_sign, pad0_, pad1_;
+    double mu, nu, rho_fast, sigma_fast, rho_slow, sigma_slow;
+    double bw_floor_fast, bw_floor_slow, guide_strength;
+    double mim_jump_prob, mim_jump_scale, temperature, rmsprop_rho, rmsprop_eps;
+} svpfo_2f_cfg;
+typedef struct svpfo_2f_state {
+    int timestep, guide_2f_initialized;
+    double guide_mean_fast, guide_mean_slow, guide_var_fast, guide_var_slow;
+    double mean_fast_prev, mean_slow_prev;   /* factor means after the previous tick (0 before the first) */
+    double vol_prev, h_mean_prev;
+} svpfo_2f_state;
+typedef struct svpfo_2f_out {
+    double loglik, pred_loglik, vol, h_mean, mean_fast, mean_slow, bw_fast, bw_slow;
+    int steps, pad_;
+} svpfo_2f_out;
+/* Per-filter EKF of the two factors with the innovation split (include/svpf_two_factor.cuh:245-300). */
+void svpfo_ekf_2f_update(const svpfo_2f_cfg* c, svpfo_2f_state* s, double y_t);
+/* One tick.  hf, hs: factor particles; hfp, hsp: their previous values; vf, vs:
+ * RMSProp states; all length n, updated in place.  Noise from the counter
+ * generator keyed by (seed, filter_id, timestep): the fast predict normal and
+ * the MIM uniform from purpose 0, the slow predict normal from purpose 2, the
+ * Langevin normals of iteration k from purposes 16 + k (fast) and 1024 + k (slow). */
+int svpfo_tick_2f(const svpfo_2f_cfg* c, svpfo_2f_state* st, double* hf, double* hs, double* hfp, double* hsp,
+                  double* vf, double* vs, double y_t, uint64_t seed, uint32_t filter_id, svpfo_2f_out* out);
+/* svpf_initialize with use_two_factor: factor particles from the stationary laws,
+ * fast and slow from the two normals of one purpose-255 draw. */
+void svpfo_init_2f(int n, double rho_fast, double sigma_fast, double rho_slow, double sigma_slow, uint64_t seed,
+                   double* hf, double* hs);
+
 /* Counter-based generator: Philox4x32-10 (Salmon et al., SC'11). */
 void svpfo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* Student-t state noise from the counter generator (particle i, tick, filter). */

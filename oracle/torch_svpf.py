@@ -49,6 +49,8 @@ def logmeanexp(x):
 def median_r2(h):
     """Lower median (rank floor((M-1)/2)) of the N(N-1)/2 pairwise squared distances."""
     n = h.shape[0]
+    if n < 2:
+        return h.sum() * 0.0  # no pair: 0, as the C oracle and the kernels
     iu = torch.triu_indices(n, n, offset=1, device=h.device)
     r2 = (h[iu[0]] - h[iu[1]]) ** 2
     k = (r2.numel() - 1) // 2

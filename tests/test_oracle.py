@@ -236,25 +236,7 @@ def test_ksd_partials_against_numerical_stein_kernel():
 
 
 # ---------------------------------------------------------------- scalar logic
-def test_adaptive_beta_known_values():
-    L = oracle.lib()
-    assert L.svpfo_adaptive_beta(0.6, 0, 3) == pytest.approx(0.3 * (0.5 + 0.5 / 3))
-    assert L.svpfo_adaptive_beta(0.01, 1, 3) == pytest.approx(0.8 * (0.5 + 0.5 * 2 / 3))
-    assert L.svpfo_adaptive_beta(0.275, 0, 2) == pytest.approx(0.55 * 0.75)
-    assert L.svpfo_adaptive_beta(0.275, 1, 2) == 1.0
-    # 0.3 * (0.5 + 0.5 * 0.1) = 0.165 stays above the 0.1 floor
-    assert L.svpfo_adaptive_beta(1e10, 0, 10) == pytest.approx(0.165)
-
-
-def test_stein_budget():
-    L = oracle.lib()
-    cfg = oracle.make_cfg(_cfg(stein_min_steps=4, stein_max_steps=12))
-    for ts, ksd, want in [(0, 0.0, 12), (9, 0.0, 12), (10, 0.0, 4), (10, 0.05, 4), (10, 0.5, 12),
-                          (10, 7.0, 12), (20, 0.275, 8)]:
-        st = oracle.State()
-        st.timestep = ts
-        st.ksd_prev = ksd
-        assert L.svpfo_stein_budget(ctypes_ref(cfg), ctypes_ref(st)) == want
+# adaptive beta and the Stein budget: property checks in tests/test_oracle_branches.py
 
 
 def ctypes_ref(x):

@@ -154,3 +154,52 @@ def test_adaptive_mu_ema_end_points():
     h, hp, v = _state(n, 10, mu=-2.0), _state(n, 11, mu=-2.0), np.zeros(n)
     _, _, _, io = oracle.v2_step(a, h, hp, v, {"mu_state": -2.0}, noise=_noise(n, 1, 12))
     assert io["mu_state"] == -2.0
+
+
+@pytest.mark.parametrize("stages,middle", [(3, [0.55]), (4, [0.3 + 0.5 / 3, 0.3 + 1.0 / 3])])
+def test_beta_ramp_recovered_from_a_single_particle(stages, middle):
+    """The fused step's annealing ramp (src/svpf_fully_fused_v2.cu:441-451: 0.3 at the
+    first stage, 1.0 at the last, 0.3 + 0.5 ai/(A-1) between), recovered from the
+    motion of one particle.  At N = 1 the operator is the score itself (K(0) = 1, no
+    repulsion); with no noise each stage moves the particle by
+    0.1 sqrt(beta) phi / sqrt(v' + 1e-6), phi = the mixture-prior score plus beta times
+    the likelihood score (central differences of scipy's Student-t).  The middle
+    stages' beta are solved for from the final position and state (one iteration per
+    stage); the graph path's linear rule (0.3 + 0.7 ai/(A-1)) would give other values."""
+    import scipy.optimize
+    import scipy.stats
+
+    mu, rho, sig, nu, y, h0 = -1.0, 0.95, 0.3, 5.0, 0.8, -0.6
+    a = v2_args(n=1, timestep=0, y_t=y, y_prev=0.0, gamma=0.0, n_stein_steps=stages, n_anneal_steps=stages,
+                use_newton=0, use_full_newton=0, temperature=0.0, use_guide=0, use_guided_predict=0,
+                use_rejuvenation=0, use_adaptive_mu=0, use_adaptive_sigma=0, mim_jump_prob=0.0, delta_rho=0.0,
+                delta_sigma=0.0, step_size=0.1, sigma_z=sig, mu=mu, rho=rho, nu=nu, lik_offset=0.0,
+                student_t_const=config.student_t_const(nu), student_t_implied_offset=config.implied_offset(nu))
+    h, hp, v = np.array([h0]), np.array([h0]), np.zeros(1)
+    nz = {"z_pred": np.zeros(1, np.float32), "u_pred": np.ones(1, np.float32),
+          "xi": np.zeros((stages, 1), np.float32)}
+    oracle.v2_step(a, h, hp, v, {"guide_mean": 0.0, "guide_var": 0.0}, noise=nz)
+    m = mu + rho * (h0 - mu)
+
+    def score(x):
+        lp = lambda q: scipy.stats.t.logpdf(y * math.exp(-q / 2), nu) - q / 2
+        return (lp(x + 1e-5) - lp(x - 1e-5)) / 2e-5
+
+    def run(betas):
+        x, vv = m, 0.0  # z = 0: the prediction is the component mean
+        for b in betas:
+            phi = float(np.clip(np.clip(-(x - m) / sig ** 2, -10, 10) + b * score(x), -10, 10))
+            vv = 0.9 * vv + 0.1 * phi * phi
+            x = x + 0.1 * math.sqrt(b) * phi / math.sqrt(vv + 1e-6)
+        return x, vv
+
+    if stages == 3:
+        b = scipy.optimize.brentq(lambda q: run([0.3, q, 1.0])[0] - h[0], 0.31, 0.99, xtol=1e-12)
+        got = [b]
+    else:
+        sol = scipy.optimize.fsolve(lambda q: [run([0.3, q[0], q[1], 1.0])[0] - h[0],
+                                               run([0.3, q[0], q[1], 1.0])[1] - v[0]], [0.5, 0.7], xtol=1e-12)
+        got = list(sol)
+    assert np.allclose(got, middle, atol=1e-5), got
+    x, vv = run([0.3] + list(middle) + [1.0])
+    assert h[0] == pytest.approx(x, abs=1e-7) and v[0] == pytest.approx(vv, abs=1e-7)

@@ -6,6 +6,7 @@ the 2-D IMQ kernel, the bandwidths against numpy sorts.  CPU only."""
 import math
 
 import numpy as np
+import pytest
 import torch
 
 from oracle import torch_svpf2d
@@ -95,3 +96,59 @@ def test_tick_bandwidths_and_shapes():
     h = 0.0 + (-1.0) + xp[:, 0] + xp[:, 1]
     ref = np.log(np.mean(np.exp(log_w(h, torch.tensor(0.4, dtype=dt), th[5], student_t_const(th[5])).numpy())))
     assert abs(pl.item() - ref) < 1e-12
+
+
+def test_single_particle_tick_is_a_hand_assembled_newton_step():
+    """N = 1: K(0) = 1 and the repulsion vanishes, so each iteration moves the particle by
+    eff phi / sqrt(v' + eps) with phi = 0.7 C^-1 g, g the AR(1) prior score (one mixture
+    component: -D (x - m)) plus beta times the likelihood score (autograd of the Student-t
+    log density), C = D + c 1 1^T with c the negative likelihood curvature (autograd),
+    inverted by numpy."""
+    th = theta()
+    mu, rf, sf, rs, ss, nu = (t.item() for t in th)
+    S = 3
+    cfg = dict(torch_svpf2d.DEFAULT_CFG, S=S, temperature=0.0)
+    x0 = torch.tensor([[0.2, -0.1]], dtype=dt)
+    z = torch.tensor([[0.4, -1.1]], dtype=dt)
+    y = torch.tensor(0.9, dtype=dt)
+    xn, vn, _ = torch_svpf2d.tick(x0, torch.zeros_like(x0), y, z, torch.zeros(S, 1, 2, dtype=dt), th, cfg)
+    m = np.array([rf * 0.2, rs * -0.1])
+    x = m + np.array([sf, ss]) * np.array([0.4, -1.1])
+    D = np.diag([1 / sf ** 2, 1 / ss ** 2])
+    v = np.zeros(2)
+    for k in range(S):
+        beta = 0.3 + 0.7 * k / (S - 1)
+        hr = torch.tensor(mu + x[0] + x[1], dtype=dt, requires_grad=True)
+        lw = log_w(hr, y, th[5], student_t_const(th[5]))
+        (gl,) = torch.autograd.grad(lw, hr, create_graph=True)
+        (hl,) = torch.autograd.grad(gl, hr)
+        g = np.clip(-D @ (x - m) + beta * gl.item(), -10, 10)
+        C = D - hl.item() * np.ones((2, 2))
+        phi = 0.7 * np.linalg.inv(C) @ g
+        v = 0.9 * v + 0.1 * phi * phi
+        x = x + 0.1 * math.sqrt(beta) * phi / np.sqrt(v + 1e-6)
+    assert np.allclose(xn.numpy()[0], x, rtol=0, atol=1e-8)  # the 1e-8 guards of the restatement
+    assert np.allclose(vn.numpy()[0], v, rtol=0, atol=1e-8)
+
+
+def test_loss_gradient_matches_central_differences():
+    """The summed predictive log-likelihood of a short run: autograd against FP64 central
+    differences (step 1e-6) in each of the six parameters."""
+    rng = np.random.default_rng(4)
+    n, S, T = 6, 3, 3
+    cfg = dict(torch_svpf2d.DEFAULT_CFG, S=S, temperature=0.5)
+    x0 = torch.tensor(rng.normal(size=(n, 2)) * [0.4, 0.2], dtype=dt)
+    z = torch.tensor(rng.normal(size=(T, n, 2)), dtype=dt)
+    xi = torch.tensor(rng.normal(size=(T, S, n, 2)), dtype=dt)
+    y = torch.tensor([0.3, -1.2, 0.5], dtype=dt)
+    base = [-1.0, 0.8, 0.3, 0.97, 0.12, 6.0]
+    th = [torch.tensor(v, dtype=dt, requires_grad=True) for v in base]
+    pl, _, _ = torch_svpf2d.run(th, y, z, xi, x0, cfg)
+    loss = -pl.sum()
+    grads = torch.autograd.grad(loss, th)
+    for k in range(6):
+        e = 1e-6
+        hi = [torch.tensor(v + (e if i == k else 0.0), dtype=dt) for i, v in enumerate(base)]
+        lo = [torch.tensor(v - (e if i == k else 0.0), dtype=dt) for i, v in enumerate(base)]
+        fd = (-torch_svpf2d.run(hi, y, z, xi, x0, cfg)[0].sum() + torch_svpf2d.run(lo, y, z, xi, x0, cfg)[0].sum()) / (2 * e)
+        assert grads[k].item() == pytest.approx(fd.item(), rel=1e-5, abs=1e-7), k
