@@ -279,6 +279,12 @@ int svpf_set_scalar_state(SVPFState* state, float h_mean_prev, float bw_sq_ema, 
 /* No reference counterpart: per-tick diagnostics of the last tick
  * (loglik, vol, h_mean, ksd, bandwidth, predictive loglik, steps, ell^2). */
 int svpf_get_last_outputs(const SVPFState* state, float out8[8]);
+/* The factor particles of a two-factor filter (include/svpf.cuh:235-246 keeps
+ * them as the device arrays d_h_fast / d_h_slow of the optimized backend):
+ * copied to host arrays of n_particles floats, either may be NULL.  The step
+ * also writes h = mu + h_fast + h_slow into state->h, which svpf_get_particles
+ * reads. */
+int svpf_get_two_factor_particles(const SVPFState* state, float* h_fast, float* h_slow);
 
 /* No reference counterpart: shape of the cooperative multi-workgroup kernel
  * that serves one filter when use_persistent_kernel is set (the default for

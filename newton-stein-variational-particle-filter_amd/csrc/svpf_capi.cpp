@@ -94,6 +94,12 @@ struct svpf_impl {
     float* dgrid_ws = nullptr;
     int captured_path = -1, captured_smax = -1;
     unsigned captured_ext = 0;
+    // coordinate-wise two-factor step: [6][n] f | s | f_prev | s_prev | v_f | v_s
+    // (allocated for filters within the single-workgroup capacity) and the factors'
+    // particle means after the last tick
+    float* d2f = nullptr;
+    float mean_fast_prev = 0.0f, mean_slow_prev = 0.0f;
+    bool captured_2f = false;
 
     svpf_fparams* hp() { return reinterpret_cast<svpf_fparams*>(hblock + kBlockParams); }
     svpf_fstate* hs() { return reinterpret_cast<svpf_fstate*>(hblock + kBlockFstate); }
@@ -221,8 +227,17 @@ void pack_params(const SVPFState* s, const SVPFParams* p, svpf_fparams* o) {
     if (s->stein_repulsive_sign == SVPF_STEIN_SIGN_PAPER) f |= SVPF_F_PAPER_SIGN;
     if (s->use_heun) f |= SVPF_F_HEUN;
     if (s->use_student_t_state) f |= SVPF_F_STUDENT_STATE;
+    if (s->use_two_factor) f |= SVPF_F_TWO_FACTOR;
     o->nu_state = s->nu_state;
     o->coop_spin_limit = s->impl->coop_spin_limit;
+    o->rho_fast = s->rho_fast;
+    o->sigma_fast = s->sigma_fast;
+    o->rho_slow = s->rho_slow;
+    o->sigma_slow = s->sigma_slow;
+    o->bw_floor_fast = s->bw_floor_fast;
+    o->bw_floor_slow = s->bw_floor_slow;
+    o->guide_strength = s->guide_strength;
+    o->n_stein = s->n_stein_steps;
     o->n = s->n_particles;
     o->flags = f;
     o->n_anneal = s->n_anneal_steps;
@@ -285,6 +300,13 @@ void pack_fstate(const SVPFState* s, float y_prev, svpf_fstate* o) {
     o->y_prev = y_prev;
     o->epoch = I->epoch;
     o->med_r2 = I->med_r2;
+    o->g2f_mean_fast = s->guide_mean_fast;
+    o->g2f_mean_slow = s->guide_mean_slow;
+    o->g2f_var_fast = s->guide_var_fast;
+    o->g2f_var_slow = s->guide_var_slow;
+    o->g2f_initialized = s->guide_2f_initialized;
+    o->mean_fast_prev = I->mean_fast_prev;
+    o->mean_slow_prev = I->mean_slow_prev;
 }
 
 void unpack_fstate(SVPFState* s, const svpf_fstate* o) {
@@ -305,6 +327,13 @@ void unpack_fstate(SVPFState* s, const svpf_fstate* o) {
     I->return_var = o->return_var;
     I->epoch = o->epoch;
     I->med_r2 = o->med_r2;
+    s->guide_mean_fast = o->g2f_mean_fast;
+    s->guide_mean_slow = o->g2f_mean_slow;
+    s->guide_var_fast = o->g2f_var_fast;
+    s->guide_var_slow = o->g2f_var_slow;
+    s->guide_2f_initialized = o->g2f_initialized;
+    I->mean_fast_prev = o->mean_fast_prev;
+    I->mean_slow_prev = o->mean_slow_prev;
 }
 
 void fill_desc(SVPFState* s, svpf_fdesc* d, const float* y, float* out, int tick0) {
@@ -333,7 +362,12 @@ void fill_desc(SVPFState* s, svpf_fdesc* d, const float* y, float* out, int tick
 }
 
 int check_supported(SVPFState* s) {
-    if (s->use_two_factor) return SVPF_RC_UNSUPPORTED;
+    if (s->use_two_factor) {
+        // one workgroup holds the filter; its noise comes from the counter generator
+        const svpf_impl* I = s->impl;
+        if (I->tile < 0 || !I->d2f || I->z_pred) return SVPF_RC_UNSUPPORTED;
+        if (s->n_stein_steps < 1 || (s->use_annealing && s->n_anneal_steps < 1)) return SVPF_RC_INVALID_ARGUMENT;
+    }
     if (s->n_particles != s->impl->n) return SVPF_RC_INVALID_ARGUMENT;
     if (s->bandwidth_mode < 0 || s->bandwidth_mode > 2) return SVPF_RC_INVALID_ARGUMENT;
     if (s->n_anneal_steps < 1 && s->use_annealing) return SVPF_RC_INVALID_ARGUMENT;
@@ -346,7 +380,7 @@ int check_supported(SVPFState* s) {
 // for it (use_persistent_kernel) and the features it implements suffice.
 bool coop_applies(const SVPFState* s) {
     const svpf_impl* I = s->impl;
-    return s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop && !s->use_heun &&
+    return !s->use_two_factor && s->use_persistent_kernel && I->coop_ok && !s->use_ksd_early_stop && !s->use_heun &&
            !s->use_student_t_state;
 }
 
@@ -360,7 +394,7 @@ constexpr int kGridAutoMinN = 1024;
 constexpr int kCoopMinN = 256;
 bool use_grid(const SVPFState* s) {
     const svpf_impl* I = s->impl;
-    if (!I->dgrid_ws) return false;
+    if (!I->dgrid_ws || s->use_two_factor) return false;
     if (I->tile < 0 || I->grid_mode == SVPF_GRID_ON) return true;
     if (I->grid_mode == SVPF_GRID_OFF) return false;
     if (I->n >= kGridAutoMinN) return true;
@@ -380,6 +414,8 @@ int step_path(const SVPFState* s) {
 
 hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
     svpf_impl* I = s->impl;
+    if (s->use_two_factor)
+        return svpf_launch_2f(I->bs, I->tile, I->dp(), I->ds(), I->dd(), I->d2f, I->d_h(), I->n, t0, t1, cs);
     if (use_grid(s))
         return svpf_launch_grid(I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->d_lw(), I->dgrid_ws,
                                 I->n, s->stein_max_steps, grid_ext(s), t0, t1, cs);
@@ -403,7 +439,10 @@ hipError_t enqueue_tick(SVPFState* s, hipStream_t cs) {
 }
 
 // RTS-style smoothing of summary statistics (src/svpf_optimized_graph.cu:619-699).
-void smooth_backward(SVPFState* st, float h_mean_new, float h_var_new, float y_t, const SVPFParams* p) {
+// mu_state: the adaptive mean before this tick's update, which the reference applies after the
+// smoothing (src/svpf_optimized_graph.cu:1366-1386).
+void smooth_backward(SVPFState* st, float h_mean_new, float h_var_new, float y_t, const SVPFParams* p,
+                     float mu_state) {
     if (!st->use_smoothing) return;
     int k = std::min(std::max(st->smooth_lag, 1), SVPF_SMOOTH_MAX_LAG);
     int head = st->smooth_head;
@@ -413,7 +452,7 @@ void smooth_backward(SVPFState* st, float h_mean_new, float h_var_new, float y_t
     st->smooth_head = (head + 1) % k;
     if (st->timestep < k) return;
     float rho = p->rho;
-    float mu = st->use_adaptive_mu ? st->mu_state : p->mu;
+    float mu = st->use_adaptive_mu ? mu_state : p->mu;
     float s2 = p->sigma_z * p->sigma_z;
     for (int lag = 1; lag < k; ++lag) {
         int ic = (st->smooth_head - lag - 1 + k) % k;
@@ -440,16 +479,21 @@ float smoothed_output(SVPFState* st, float raw) {
 }
 
 // Host tail of one tick after the record came back.
-void finish_tick(SVPFState* s, const SVPFParams* p, float y_t, const float* rec, float* ll, float* vol, float* hm) {
+void finish_tick(SVPFState* s, const SVPFParams* p, float y_t, const float* rec, float* ll, float* vol, float* hm,
+                 float mu_before) {
     svpf_impl* I = s->impl;
     std::memcpy(I->out_last, rec, sizeof(I->out_last));
     float h_mean = rec[SVPF_O_HMEAN], bw = rec[SVPF_O_BW];
     // smoothing reads the pre-increment timestep, as the reference does
+    // the two-factor branch returns before the smoothing (src/svpf_optimized_graph.cu:924-928)
     int ts = s->timestep;
-    s->timestep = ts - 1;
-    smooth_backward(s, h_mean, bw * bw, y_t, p);
-    float out_h = smoothed_output(s, h_mean);
-    s->timestep = ts;
+    float out_h = h_mean;
+    if (!s->use_two_factor) {
+        s->timestep = ts - 1;
+        smooth_backward(s, h_mean, bw * bw, y_t, p, mu_before);
+        out_h = smoothed_output(s, h_mean);
+        s->timestep = ts;
+    }
     if (ll) *ll = rec[SVPF_O_LOGLIK];
     if (vol) *vol = rec[SVPF_O_VOL];
     if (hm) *hm = out_h;
@@ -484,7 +528,7 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     const bool coop = use_coop(s);
     const int path = step_path(s);
     // the captured launch sequence depends on the path and, on the grid path, on stein_max_steps
-    if (I->captured && (I->captured_path != path ||
+    if (I->captured && (I->captured_path != path || I->captured_2f != (s->use_two_factor != 0) ||
                         (path == SVPF_PATH_GRID && (I->captured_smax != s->stein_max_steps ||
                                                     I->captured_ext != grid_ext(s))) ||
                         (path == SVPF_PATH_COOP && I->captured_cooperative != I->coop_cooperative)))
@@ -509,6 +553,7 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
                     I->captured_smax = s->stein_max_steps;
                     I->captured_cooperative = I->coop_cooperative;
                     I->captured_ext = grid_ext(s);
+                    I->captured_2f = s->use_two_factor != 0;
                 } else if (g) {
                     hipGraphDestroy(g);
                 }
@@ -545,8 +590,9 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
         set_err(s, I->hs()->status == 2 ? SVPF_RC_TIMEOUT : SVPF_RC_UNSUPPORTED);
         return;
     }
+    const float mu_before = s->mu_state;
     unpack_fstate(s, I->hs());
-    finish_tick(s, p, y_t, I->hout(), ll, vol, hm);
+    finish_tick(s, p, y_t, I->hout(), ll, vol, hm, mu_before);
     set_err(s, SVPF_RC_OK);
 }
 
@@ -625,6 +671,7 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
     ok = ok && hipHostMalloc(&I->hblock, kBlockBytes, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipMalloc(&I->dpool, 5 * (size_t)n_particles * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&I->dorder, sizeof(int)) == hipSuccess;
+    if (!grid_only) ok = ok && hipMalloc(&I->d2f, 6 * (size_t)n_particles * sizeof(float)) == hipSuccess;
     // exchange words of the cooperative path: [2][4][n] records, [2][n] final positions,
     // [2][G][8] partials with G <= n / 8 + 1
     const size_t xwords = 10 * (size_t)n_particles + 16 * ((size_t)n_particles / 8 + 2);
@@ -636,6 +683,7 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
         ok = ok && hipMemset(I->dpool, 0, 5 * (size_t)n_particles * sizeof(float)) == hipSuccess;
         ok = ok && hipMemset(I->dblock, 0, kBlockBytes) == hipSuccess;
         ok = ok && hipMemset(I->xbuf, 0, xwords * sizeof(uint64_t)) == hipSuccess;
+        if (I->d2f) ok = ok && hipMemset(I->d2f, 0, 6 * (size_t)n_particles * sizeof(float)) == hipSuccess;
     }
     if (ok && !grid_only) choose_coop_shape(I, -1, -1);
     if (!ok) {
@@ -715,7 +763,19 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
     s->ksd_prev = 1e10f;
     s->stein_steps_used = n_stein_steps;
     s->use_heun = 0;
+    // src/svpf_optimized_graph.cu:260-272
     s->use_two_factor = 0;
+    s->rho_fast = 0.90f;
+    s->sigma_fast = 0.15f;
+    s->rho_slow = 0.99f;
+    s->sigma_slow = 0.05f;
+    s->bw_floor_fast = 0.01f;
+    s->bw_floor_slow = 0.05f;
+    s->guide_mean_fast = 0.0f;
+    s->guide_mean_slow = 0.0f;
+    s->guide_var_fast = 0.0f;
+    s->guide_var_slow = 0.0f;
+    s->guide_2f_initialized = 0;
     s->use_smoothing = 0;
     s->smooth_lag = 3;
     s->smooth_output_lag = 1;
@@ -745,6 +805,7 @@ void svpf_destroy(SVPFState* s) {
         if (I->dorder) hipFree(I->dorder);
         if (I->xbuf) hipFree(I->xbuf);
         if (I->dgrid_ws) hipFree(I->dgrid_ws);
+        if (I->d2f) hipFree(I->d2f);
         if (I->dseq_out) hipFree(I->dseq_out);
         if (I->dseq_y) hipFree(I->dseq_y);
         delete I;
@@ -786,6 +847,22 @@ void svpf_initialize(SVPFState* s, const SVPFParams* p, unsigned long long seed)
     s->ksd_prev = 1e10f;
     s->stein_steps_used = s->n_stein_steps;
     I->med_r2 = 0.0f;
+    if (s->use_two_factor && I->d2f) {
+        // src/svpf_optimized_graph.cu:414-448
+        e = svpf_launch_2f_init(I->d2f, I->n, s->rho_fast, s->sigma_fast, s->rho_slow, s->sigma_slow, seed, s->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+        if (e != hipSuccess) {
+            set_err(s, SVPF_RC_HIP);
+            return;
+        }
+        s->guide_2f_initialized = 0;
+        s->guide_mean_fast = 0.0f;
+        s->guide_mean_slow = 0.0f;
+        s->guide_var_fast = s->sigma_fast * s->sigma_fast / (1.0f - s->rho_fast * s->rho_fast + 1e-6f);
+        s->guide_var_slow = s->sigma_slow * s->sigma_slow / (1.0f - s->rho_slow * s->rho_slow + 1e-6f);
+        I->mean_fast_prev = 0.0f;
+        I->mean_slow_prev = 0.0f;
+    }
     svpf_graph_invalidate(s);
     I->initialized = true;
     set_err(s, SVPF_RC_OK);
@@ -839,6 +916,17 @@ void svpf_run_sequence(SVPFState* s, const float* obs, int T, const SVPFParams* 
         set_err(s, SVPF_RC_OK);
         return;
     }
+    if (s->use_smoothing && s->use_adaptive_mu && !s->use_two_factor) {
+        // the smoothing of each tick reads that tick's adaptive mean, which one device run
+        // does not report: step tick by tick
+        float yp = 0.0f;  // src/svpf_optimized_graph.cu:1414-1424
+        for (int t = 0; t < T; ++t) {
+            step_impl(s, obs[t], yp, p, ll ? &ll[t] : nullptr, vol ? &vol[t] : nullptr, nullptr, true, 0, false);
+            if (I->last_error != SVPF_RC_OK) return;
+            yp = obs[t];
+        }
+        return;
+    }
     if (I->dseq_y_cap < T) {
         if (I->dseq_y) hipFree(I->dseq_y);
         I->dseq_y = nullptr;
@@ -867,7 +955,7 @@ void svpf_run_sequence(SVPFState* s, const float* obs, int T, const SVPFParams* 
     for (int t = 0; t < T; ++t) {
         const float* r = &rec[(size_t)t * SVPF_OUTW];
         s->timestep = t0 + t + 1;
-        finish_tick(s, p, obs[t], r, ll ? &ll[t] : nullptr, vol ? &vol[t] : nullptr, nullptr);
+        finish_tick(s, p, obs[t], r, ll ? &ll[t] : nullptr, vol ? &vol[t] : nullptr, nullptr, s->mu_state);
     }
     s->timestep = tend;
     set_err(s, SVPF_RC_OK);
@@ -1026,6 +1114,16 @@ int svpf_set_scalar_state(SVPFState* s, float h_mean_prev, float bw_sq_ema, floa
     s->impl->return_ema = return_ema;
     s->impl->return_var = return_var;
     return set_err(s, SVPF_RC_OK);
+}
+
+int svpf_get_two_factor_particles(const SVPFState* s, float* h_fast, float* h_slow) {
+    if (!s || !s->impl || !s->impl->d2f) return SVPF_RC_INVALID_ARGUMENT;
+    const svpf_impl* I = s->impl;
+    const size_t nb = (size_t)I->n * sizeof(float);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess && h_fast) e = hipMemcpy(h_fast, I->d2f, nb, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && h_slow) e = hipMemcpy(h_slow, I->d2f + I->n, nb, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? SVPF_RC_OK : SVPF_RC_HIP;
 }
 
 int svpf_get_last_outputs(const SVPFState* s, float out8[8]) {

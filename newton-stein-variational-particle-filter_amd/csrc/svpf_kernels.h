@@ -104,3 +104,18 @@ hipError_t svpf_launch_diag(int mode, const float* h, const float* h2, const flo
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+// The coordinate-wise two-factor step for one filter of n particles (one
+// workgroup; the single-workgroup tile table), over ticks [t0, t1).  x2f:
+// [6][n] = f | s | f_prev | s_prev | v_f | v_s; h (may be null): mu + f + s
+// written at the end of the launch.
+hipError_t svpf_launch_2f(int bs, int tile, const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, float* x2f,
+                          float* h, int n, int t0, int t1, hipStream_t stream);
+hipError_t svpf_launch_2f_init(float* x2f, int n, float rho_f, float sigma_f, float rho_s, float sigma_s,
+                               uint64_t seed, hipStream_t stream);
+#ifdef __cplusplus
+}
+#endif

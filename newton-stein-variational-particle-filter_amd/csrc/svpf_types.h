@@ -39,6 +39,7 @@ enum {
     SVPF_F_PAPER_SIGN = 1u << 19,
     SVPF_F_HEUN = 1u << 20,           // Heun predictor-corrector Stein step
     SVPF_F_STUDENT_STATE = 1u << 21,  // Student-t state noise and transition prior (nu_state)
+    SVPF_F_TWO_FACTOR = 1u << 22,     // the coordinate-wise two-factor step (its own kernel)
 };
 
 // Bandwidth rules.
@@ -65,7 +66,11 @@ typedef struct svpf_fparams {
     float nu_state;
     int32_t coop_spin_limit;  // polls before a cooperative-kernel exchange times out (0: 2^20;
                               // negative: a forced time-out after that many polls)
-    float pad1[20];
+    // the coordinate-wise two-factor step (SVPF_F_TWO_FACTOR)
+    float rho_fast, sigma_fast, rho_slow, sigma_slow;
+    float bw_floor_fast, bw_floor_slow, guide_strength;
+    int32_t n_stein;          // two-factor step: Stein iterations per annealing stage
+    float pad1[12];
 } svpf_fparams;
 
 // Per-filter scalar state carried between ticks (host fields of SVPFState and
@@ -82,7 +87,10 @@ typedef struct svpf_fstate {
     float last_ksd, last_pred_loglik, last_guide_strength;
     uint32_t epoch;            // last exchange epoch of the cooperative kernel
     float med_r2;              // last median pairwise r^2 (seeds the next tick's selection)
-    float pad2[7];
+    // two-factor step: the factors' EKF guide and their particle means after the last tick
+    float g2f_mean_fast, g2f_mean_slow, g2f_var_fast, g2f_var_slow;
+    float mean_fast_prev, mean_slow_prev;
+    int32_t g2f_initialized;
 } svpf_fstate;
 
 // Per-tick output record (8 floats): written to out[t * SVPF_OUTW + k].
@@ -112,7 +120,12 @@ typedef struct svpf_fdesc {
 
 // Noise purposes: third word of the counter-generator counter.
 // SVPF_PURPOSE_STATE_T0 + k: normal k of the chi-squared sum of a Student-t state draw.
-enum { SVPF_PURPOSE_PREDICT = 0, SVPF_PURPOSE_REJUV = 1, SVPF_PURPOSE_INIT = 255, SVPF_PURPOSE_SVLD0 = 16,
+// Two-factor step: the fast factor's predict normal and MIM uniform come from
+// SVPF_PURPOSE_PREDICT, the slow factor's normal from SVPF_PURPOSE_PREDICT_SLOW;
+// the Langevin normals of iteration k from SVPF_PURPOSE_SVLD0 + k (fast) and
+// SVPF_PURPOSE_SVLD_SLOW0 + k (slow); its initial draw from SVPF_PURPOSE_INIT_2F.
+enum { SVPF_PURPOSE_PREDICT = 0, SVPF_PURPOSE_REJUV = 1, SVPF_PURPOSE_PREDICT_SLOW = 2, SVPF_PURPOSE_INIT = 255,
+       SVPF_PURPOSE_INIT_2F = 254, SVPF_PURPOSE_SVLD0 = 16, SVPF_PURPOSE_SVLD_SLOW0 = 1024,
        SVPF_PURPOSE_STATE_T0 = 128 };
 
 // ---------------------------------------------------------------------------

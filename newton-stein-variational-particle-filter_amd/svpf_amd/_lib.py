@@ -102,6 +102,7 @@ EXPORTED = (
     "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
     "svpf_set_grid_path", "svpf_get_path", "svpf_fully_fused_step_v2", "svpf_fully_fused_v2_supported",
     "svpf_coop_launch_is_cooperative", "svpf_set_coop_spin_limit", "svpf_digamma_f32",
+    "svpf_get_two_factor_particles",
     # score diagnostics (include/svpf.h)
     "svpf_gradient_diagnostic_create", "svpf_gradient_diagnostic_destroy", "svpf_compute_nu_diagnostic",
     "svpf_compute_nu_diagnostic_simple", "svpf_compute_sigma_diagnostic_simple", "svpf_snapshot_particles",
@@ -219,6 +220,8 @@ def load(path: str | None = None):
     L.svpf_fully_fused_v2_supported.argtypes = [c_int]
     L.svpf_version.restype = ctypes.c_char_p
     L.svpf_version.argtypes = []
+    L.svpf_get_two_factor_particles.restype = c_int
+    L.svpf_get_two_factor_particles.argtypes = [SP, c_void_p, c_void_p]
     L.svpf_digamma_f32.restype = c_float
     L.svpf_digamma_f32.argtypes = [c_float]
     if path is None:

@@ -30,7 +30,10 @@ _PARAM_FLOATS = ("mu", "rho", "sigma_z", "gamma", "nu", "lik_offset", "temperatu
                  "mu_max", "sb_thr", "sb_max", "ksd_thr", "st_const", "implied_offset", "base_step")
 FPARAMS_DTYPE = np.dtype([("n", "<i4"), ("flags", "<u4"), ("n_anneal", "<i4"), ("stein_min", "<i4"),
                           ("stein_max", "<i4"), ("bandwidth_mode", "<i4"), ("pad0", "<i4", (2,))]
-                         + [(f, "<f4") for f in _PARAM_FLOATS] + [("nu_state", "<f4"), ("coop_spin_limit", "<i4"), ("pad1", "<f4", (20,))])
+                         + [(f, "<f4") for f in _PARAM_FLOATS] + [("nu_state", "<f4"), ("coop_spin_limit", "<i4")]
+                         + [(f, "<f4") for f in ("rho_fast", "sigma_fast", "rho_slow", "sigma_slow", "bw_floor_fast",
+                                                 "bw_floor_slow", "guide_strength")]
+                         + [("n_stein", "<i4"), ("pad1", "<f4", (12,))])
 FSTATE_DTYPE = np.dtype([("timestep", "<i4"), ("guide_initialized", "<i4"), ("stein_steps_used", "<i4"),
                          ("status", "<i4")]
                         + [(f, "<f4") for f in ("ksd_prev", "vol_prev", "h_mean_prev", "guide_mean", "guide_var",
@@ -38,7 +41,9 @@ FSTATE_DTYPE = np.dtype([("timestep", "<i4"), ("guide_initialized", "<i4"), ("st
                                                 "sigma_z_eff", "y_prev", "last_loglik", "last_vol", "last_h_mean",
                                                 "last_bw", "last_ksd", "last_pred_loglik", "last_guide_strength")]
                         + [("epoch", "<u4"), ("med_r2", "<f4")]
-                        + [("pad2", "<f4", (7,))])
+                        + [(f, "<f4") for f in ("g2f_mean_fast", "g2f_mean_slow", "g2f_var_fast", "g2f_var_slow",
+                                                "mean_fast_prev", "mean_slow_prev")]
+                        + [("g2f_initialized", "<i4")])
 FDESC_DTYPE = np.dtype([("off", "<i8")] + [(f, "<u8") for f in ("y", "out", "z_pred", "u_pred", "xi", "u_rej", "z_rej",
                                                                   "log_w", "h_pred", "seed")]
                        + [("xi_rows", "<i4"), ("param_idx", "<i4"), ("filter_id", "<u4"), ("tile", "<i4")])

@@ -30,6 +30,7 @@ INT_FIELDS = (
     "use_asymmetric_rho", "use_local_params", "use_guided", "use_guide",
     "use_guide_preserving", "use_adaptive_guide", "use_rejuvenation", "use_fan_mode",
     "use_adaptive_mu", "use_adaptive_sigma", "use_ksd_early_stop", "use_heun", "use_student_t_state",
+    "use_two_factor",
 )
 FLOAT_FIELDS = (
     "mu", "rho", "sigma_z", "gamma", "nu", "lik_offset", "temperature", "rmsprop_rho",
@@ -39,6 +40,7 @@ FLOAT_FIELDS = (
     "rejuv_ksd_threshold", "rejuv_prob", "rejuv_blend", "mu_process_var",
     "mu_obs_var_scale", "mu_min", "mu_max", "sigma_boost_threshold", "sigma_boost_max",
     "ksd_improvement_threshold", "nu_state",
+    "rho_fast", "sigma_fast", "rho_slow", "sigma_slow", "bw_floor_fast", "bw_floor_slow", "guide_strength",
 )
 
 # src/svpf_optimized_graph.cu:110-302 (svpf_create) with SVPFParams defaults of
@@ -59,6 +61,9 @@ REFERENCE_DEFAULTS = dict(
     mu_obs_var_scale=10.0, mu_min=-6.0, mu_max=-1.0, sigma_boost_threshold=1.0,
     sigma_boost_max=3.0, ksd_improvement_threshold=0.05,
     use_heun=0, use_student_t_state=0, nu_state=5.0,
+    # the coordinate-wise two-factor step (src/svpf_optimized_graph.cu:188, :260-266)
+    use_two_factor=0, rho_fast=0.90, sigma_fast=0.15, rho_slow=0.99, sigma_slow=0.05, bw_floor_fast=0.01,
+    bw_floor_slow=0.05, guide_strength=0.05,
 )
 
 

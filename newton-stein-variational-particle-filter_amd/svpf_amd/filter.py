@@ -183,6 +183,15 @@ class Filter:
             raise RuntimeError(f"hipMemcpy of the log-weights failed ({rc})")
         return lw
 
+    def two_factor_particles(self):
+        """(h_fast, h_slow) of a two-factor filter, read back synchronously."""
+        hf = np.zeros(self.n, dtype=np.float32)
+        hs = np.zeros(self.n, dtype=np.float32)
+        rc = self.L.svpf_get_two_factor_particles(self.sp, hf.ctypes.data, hs.ctypes.data)
+        if rc:
+            raise RuntimeError(f"svpf_get_two_factor_particles failed ({rc})")
+        return hf, hs
+
     def rmsprop_state(self):
         v = np.zeros(self.n, dtype=np.float32)
         self.L.svpf_get_rmsprop_state(self.sp, v.ctypes.data)

@@ -1,8 +1,14 @@
 """Parity of the HIP path (through the C ABI) against the FP64 oracle.
 
 Tolerances: per-particle |dh| <= 1e-4 after one tick from identical state;
-filtered log-vol (h_mean) RMSE <= 1e-4 over whole trajectories; the median
-bandwidth selection is bit-exact against a float32 sort of the same particles.
+filtered log-vol (h_mean) RMSE <= 1e-4 over whole trajectories; every column of
+the per-tick record is compared: log-likelihood and predictive log-likelihood
+to 1e-3 (absolute), vol to 1e-4, the KSD of the last Stein iteration to 2e-3
+relative (absolute below 0.05; the reference's sign of the mixed kernel
+derivative changes it by far more), the bandwidth to 1e-3 relative (a pair
+distance inherits the particles' absolute error over the whole trajectory) and the
+Stein step count exactly; the median bandwidth selection is bit-exact against
+a float32 sort of the same particles.
 """
 import ctypes
 import os
@@ -55,12 +61,23 @@ def run_oracle(cfg, y, noise, h0):
     return o, out
 
 
-def compare(rec, out, tol_hmean=1e-4, tol_ll=1e-3, tol_vol=1e-4):
+def compare(rec, out, tol_hmean=1e-4, tol_ll=1e-3, tol_vol=1e-4, tol_ksd=2e-3, tol_bw=1e-3):
+    """Per-tick device records [T, 8] against the oracle's outputs (every column the oracle reports)."""
     dh = rec[:, 2] - out["h_mean"]
     assert np.sqrt(np.mean(dh ** 2)) <= tol_hmean, np.abs(dh).max()
     assert np.max(np.abs(rec[:, 1] - out["vol"])) <= tol_vol * max(1.0, np.abs(out["vol"]).max())
     assert np.max(np.abs(rec[:, 0] - out["loglik"])) <= tol_ll
-    assert np.array_equal(rec[:, 6].astype(int), out["steps"])
+    assert np.array_equal(rec[:, 6].astype(int), np.asarray(out["steps"]).astype(int))
+    if "ksd" in out:
+        k = np.asarray(out["ksd"], dtype=np.float64)
+        err = np.abs(rec[:, 3] - k) / np.maximum(0.05, np.abs(k))
+        assert err.max() <= tol_ksd, (err.max(), k[np.argmax(err)], rec[np.argmax(err), 3])
+    bw = out.get("bw", out.get("bandwidth"))
+    if bw is not None:
+        bw = np.asarray(bw, dtype=np.float64)
+        assert np.max(np.abs(rec[:, 4] - bw) / bw) <= tol_bw
+    if "pred_loglik" in out:
+        assert np.max(np.abs(rec[:, 5] - out["pred_loglik"])) <= tol_ll
 
 
 @pytest.mark.parametrize("path", ["single", "coop"])
@@ -89,7 +106,7 @@ def test_plain_filter_trajectory_matches_golden(torch_cuda, path):
     g = np.load(os.path.join(HERE, "golden", "plain_n512_t1000.npz"))
     cfg, y, h_true, noise, h0 = plain_inputs()
     f, rec = run_gpu(torch_cuda, cfg, y, noise, h0, path=path)
-    out = {k: g[k] for k in ("h_mean", "vol", "loglik", "steps")}
+    out = {k: g[k] for k in ("h_mean", "vol", "loglik", "steps", "ksd", "bw", "pred_loglik")}
     compare(rec, out)
     assert np.sqrt(np.mean((f.particles() - g["h_final"]) ** 2)) <= 1e-4
 
@@ -197,8 +214,9 @@ def test_unsupported_and_uninitialized_fail_loudly(torch_cuda):
     with pytest.raises(RuntimeError, match="not initialized"):
         f.step(0.1, 0.0)
     f.initialize(7)
-    f.state.use_two_factor = 1  # the two-factor form is the svpf_diff2d_* entry points
-    with pytest.raises(RuntimeError, match="unsupported"):
+    f.state.use_two_factor = 1
+    f.state.n_stein_steps = 0  # the two-factor step needs at least one Stein iteration
+    with pytest.raises(RuntimeError, match="invalid"):
         f.step(0.1, 0.0)
     f.state.use_two_factor = 0
     f.step(0.1, 0.0)
@@ -291,3 +309,50 @@ def test_coop_exchange_timeout_repeats_with_a_cooperative_launch(torch_cuda):
     assert np.array_equal(f.particles(), g.particles())
     assert np.array_equal(f.last_outputs(), g.last_outputs())
     assert L.svpf_set_coop_spin_limit(f.sp, -(1 << 21)) == _lib.SVPF_RC_INVALID_ARGUMENT
+
+
+def test_initialize_draws_the_oracle_particles(torch_cuda):
+    """svpf_initialize (src/svpf_optimized_graph.cu:84-97, 349-451): particles from the counter
+    generator keyed by the seed, against the oracle's draw; the Student-t state widens them."""
+    for n, kw in ((1000, {}), (333, dict(use_student_t_state=1, nu_state=4.0))):
+        cfg = config.make_config(n=n, mu=-1.2, rho=0.95, sigma_z=0.25, **kw)
+        f = Filter(cfg)
+        f.initialize(seed=987654321)
+        want = oracle.init_particles(n, -1.2, 0.95, 0.25, 987654321, student_t_state=cfg["use_student_t_state"],
+                                     nu_state=cfg["nu_state"])
+        got = f.particles()
+        assert np.max(np.abs(got - want)) <= 1e-5 * max(1.0, np.abs(want).max())
+        assert np.array_equal(d2h(f.state.h_prev, n), got)
+        assert np.array_equal(f.rmsprop_state(), np.zeros(n, np.float32))
+        assert f.state.timestep == 0 and f.state.ksd_prev == pytest.approx(1e10)
+        f.close()
+
+
+@pytest.mark.parametrize("lag,out_lag,adaptive_mu", [(3, 1, 0), (5, 2, 0), (4, 1, 1)])
+def test_smoothing_matches_oracle(torch_cuda, lag, out_lag, adaptive_mu):
+    """use_smoothing: the step's reported h_mean is the RTS-corrected estimate output_lag
+    ticks back (src/svpf_optimized_graph.cu:619-717), fed by the tick's h_mean and
+    bandwidth^2, with the adaptive mean as it was before the tick's update."""
+    n, T = 256, 30
+    extra = dict(use_adaptive_mu=1, mu_obs_var_scale=11.0, mu_min=-4.0, mu_max=0.0) if adaptive_mu else {}
+    cfg = config.plain_config(n=n, **extra)
+    y, _ = data.sv_series(T, seed=41)
+    noise = data.predraw_noise(T, n, 5, seed=42)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=43)
+    f = Filter(cfg)
+    f.set_particles(h0)
+    f.set_noise(dev(torch_cuda, noise["z_pred"]), dev(torch_cuda, noise["xi"]))
+    f.state.use_smoothing = 1
+    f.state.smooth_lag = lag
+    f.state.smooth_output_lag = out_lag
+    o = oracle.OracleFilter(cfg, h0)
+    sm = oracle.Smoother(lag, out_lag)
+    yp = 0.0
+    for t in range(T):
+        hm = f.step(float(y[t]), yp)[2]
+        mu_before = o.st.mu_state if adaptive_mu else cfg["mu"]
+        r = o.tick(float(y[t]), yp, z_pred=noise["z_pred"][t], xi=noise["xi"][t])
+        want = sm.step(t, r["h_mean"], r["bandwidth"] ** 2, float(y[t]), mu_before, cfg["rho"], cfg["sigma_z"])
+        assert abs(hm - want) <= 1e-4, (t, hm, want)
+        yp = float(y[t])
+    f.close()
