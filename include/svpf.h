@@ -66,6 +66,7 @@ extern "C" {
 #define SVPF_RC_NOT_INITIALIZED 4
 #define SVPF_RC_OUT_OF_MEMORY 5
 #define SVPF_RC_TIMEOUT 6 /* a cooperative kernel's cross-workgroup exchange timed out */
+#define SVPF_RC_TOO_LARGE 7 /* above the fixed scratch of svpf_run_sequence_device_noalloc */
 
 /* include/svpf.cuh:257-262 */
 typedef struct {
@@ -237,6 +238,17 @@ void svpf_run_sequence(SVPFState* state, const float* h_observations, int T, con
 /* include/svpf.cuh:701-708 -- a device loop: one launch advances all T ticks. */
 void svpf_run_sequence_device(SVPFState* state, const float* d_observations, int T, const SVPFParams* params,
                               float* d_loglik_out, float* d_vol_out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory during a call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): svpf_run_sequence_device with the per-tick
+ * records in a fixed scratch array compiled into the library (one per GPU, held
+ * until the process ends) instead of a per-state buffer grown with hipMalloc.
+ * The call allocates nothing; T above SVPF_NOALLOC_MAX_TICKS is rejected with
+ * SVPF_RC_TOO_LARGE.  Calls that overlap on one GPU share that array and race:
+ * issue them from one host thread, one at a time per GPU. */
+#define SVPF_NOALLOC_MAX_TICKS 32768
+void svpf_run_sequence_device_noalloc(SVPFState* state, const float* d_observations, int T,
+                                      const SVPFParams* params, float* d_loglik_out, float* d_vol_out);
 /* include/svpf.cuh:767-775 */
 void svpf_step_adaptive(SVPFState* state, float y_t, float y_prev, const SVPFParams* params, float* h_loglik_out,
                         float* h_vol_out, float* h_mean_out);

@@ -596,22 +596,28 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
     set_err(s, SVPF_RC_OK);
 }
 
-// T ticks in one launch; records land in I->dseq_out[T][8].
-int run_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, float y_prev0) {
+// T ticks in one launch; records land in records[T][8] (null: the state's own
+// buffer I->dseq_out, grown with hipMalloc when T exceeds it).
+int run_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, float y_prev0,
+               float* records = nullptr) {
     svpf_impl* I = s->impl;
     if (!I->initialized) return SVPF_RC_NOT_INITIALIZED;
     int rc = check_supported(s);
     if (rc) return rc;
     if (T <= 0) return SVPF_RC_OK;
-    if (I->dseq_cap < T) {
-        if (I->dseq_out) hipFree(I->dseq_out);
-        I->dseq_out = nullptr;
-        if (hipMalloc(&I->dseq_out, (size_t)T * SVPF_OUTW * sizeof(float)) != hipSuccess) return SVPF_RC_OUT_OF_MEMORY;
-        I->dseq_cap = T;
+    if (!records) {
+        if (I->dseq_cap < T) {
+            if (I->dseq_out) hipFree(I->dseq_out);
+            I->dseq_out = nullptr;
+            if (hipMalloc(&I->dseq_out, (size_t)T * SVPF_OUTW * sizeof(float)) != hipSuccess)
+                return SVPF_RC_OUT_OF_MEMORY;
+            I->dseq_cap = T;
+        }
+        records = I->dseq_out;
     }
     pack_params(s, p, I->hp());
     pack_fstate(s, y_prev0, I->hs());
-    fill_desc(s, I->hd(), d_obs, I->dseq_out, s->timestep);
+    fill_desc(s, I->hd(), d_obs, records, s->timestep);
     hipStream_t cs = s->stream;
     const bool coop = use_coop(s);
     hipError_t e = hipSuccess;
@@ -961,33 +967,45 @@ void svpf_run_sequence(SVPFState* s, const float* obs, int T, const SVPFParams* 
     set_err(s, SVPF_RC_OK);
 }
 
-void svpf_run_sequence_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, float* d_ll,
-                              float* d_vol) {
+}  // extern "C"
+
+// svpf_run_sequence_device with the per-tick records in `records` (null: the
+// state's own buffer).  Also the body of the allocation-free variant.
+void svpf_internal_run_sequence_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, float* d_ll,
+                                       float* d_vol, float* records) {
     if (!s || !s->impl) return;
     svpf_impl* I = s->impl;
     if (!d_obs || !p || T < 0) {
         set_err(s, SVPF_RC_INVALID_ARGUMENT);
         return;
     }
-    int rc = run_device(s, d_obs, T, p, 0.0f);
+    int rc = run_device(s, d_obs, T, p, 0.0f, records);
     if (rc) {
         set_err(s, rc);
         return;
     }
+    if (!records) records = I->dseq_out;
     hipError_t e = hipSuccess;
-    if (d_ll)
-        e = hipMemcpy2DAsync(d_ll, sizeof(float), I->dseq_out + SVPF_O_LOGLIK, SVPF_OUTW * sizeof(float),
-                             sizeof(float), T, hipMemcpyDeviceToDevice, s->stream);
-    if (e == hipSuccess && d_vol)
-        e = hipMemcpy2DAsync(d_vol, sizeof(float), I->dseq_out + SVPF_O_VOL, SVPF_OUTW * sizeof(float),
-                             sizeof(float), T, hipMemcpyDeviceToDevice, s->stream);
+    if (d_ll && T > 0)
+        e = hipMemcpy2DAsync(d_ll, sizeof(float), records + SVPF_O_LOGLIK, SVPF_OUTW * sizeof(float), sizeof(float), T,
+                             hipMemcpyDeviceToDevice, s->stream);
+    if (e == hipSuccess && d_vol && T > 0)
+        e = hipMemcpy2DAsync(d_vol, sizeof(float), records + SVPF_O_VOL, SVPF_OUTW * sizeof(float), sizeof(float), T,
+                             hipMemcpyDeviceToDevice, s->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e == hipSuccess && T > 0) {
         float last[SVPF_OUTW];
-        e = hipMemcpy(last, I->dseq_out + (size_t)(T - 1) * SVPF_OUTW, sizeof(last), hipMemcpyDeviceToHost);
+        e = hipMemcpy(last, records + (size_t)(T - 1) * SVPF_OUTW, sizeof(last), hipMemcpyDeviceToHost);
         if (e == hipSuccess) std::memcpy(I->out_last, last, sizeof(last));
     }
     set_err(s, e == hipSuccess ? SVPF_RC_OK : SVPF_RC_HIP);
+}
+
+extern "C" {
+
+void svpf_run_sequence_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, float* d_ll,
+                              float* d_vol) {
+    svpf_internal_run_sequence_device(s, d_obs, T, p, d_ll, d_vol, nullptr);
 }
 
 bool svpf_graph_is_captured(SVPFState* s) { return s && s->impl && s->impl->captured; }
@@ -1063,6 +1081,7 @@ const char* svpf_last_error_string(const SVPFState* s) {
         case SVPF_RC_NOT_INITIALIZED: return "filter not initialized";
         case SVPF_RC_OUT_OF_MEMORY: return "out of memory";
         case SVPF_RC_TIMEOUT: return "cooperative exchange timed out";
+        case SVPF_RC_TOO_LARGE: return "request exceeds the allocation-free entry point's fixed scratch";
         default: return "unknown";
     }
 }

@@ -79,6 +79,8 @@ SVPF_RC_HIP = 3
 SVPF_RC_NOT_INITIALIZED = 4
 SVPF_RC_OUT_OF_MEMORY = 5
 SVPF_RC_TIMEOUT = 6
+SVPF_RC_TOO_LARGE = 7
+NOALLOC_MAX_TICKS = 32768
 
 
 
@@ -102,7 +104,7 @@ EXPORTED = (
     "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
     "svpf_set_grid_path", "svpf_get_path", "svpf_fully_fused_step_v2", "svpf_fully_fused_v2_supported",
     "svpf_coop_launch_is_cooperative", "svpf_set_coop_spin_limit", "svpf_digamma_f32",
-    "svpf_get_two_factor_particles",
+    "svpf_get_two_factor_particles", "svpf_run_sequence_device_noalloc",
     # score diagnostics (include/svpf.h)
     "svpf_gradient_diagnostic_create", "svpf_gradient_diagnostic_destroy", "svpf_compute_nu_diagnostic",
     "svpf_compute_nu_diagnostic_simple", "svpf_compute_sigma_diagnostic_simple", "svpf_snapshot_particles",
@@ -161,6 +163,8 @@ def load(path: str | None = None):
     L.svpf_run_sequence.argtypes = [SP, c_void_p, c_int, PP, c_void_p, c_void_p]
     L.svpf_run_sequence_device.restype = None
     L.svpf_run_sequence_device.argtypes = [SP, c_void_p, c_int, PP, c_void_p, c_void_p]
+    L.svpf_run_sequence_device_noalloc.restype = None
+    L.svpf_run_sequence_device_noalloc.argtypes = [SP, c_void_p, c_int, PP, c_void_p, c_void_p]
     for name in ("svpf_step_adaptive", "svpf_step_graph"):
         f = getattr(L, name)
         f.restype = None
