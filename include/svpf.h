@@ -316,6 +316,10 @@ int svpf_get_coop_shape(const SVPFState* state, int* lanes_per_row, int* workgro
  * testing, at least -2^20). */
 int svpf_coop_launch_is_cooperative(const SVPFState* state);
 int svpf_set_coop_spin_limit(SVPFState* state, int polls);
+/* Failure-path testing: the given workgroup of the cooperative kernel (-1: none)
+ * gives up the waits of a launch's final exchange after 4 polls, after publishing
+ * its rows, while the others complete; the launch then commits nothing. */
+int svpf_set_coop_fault(SVPFState* state, int workgroup);
 
 /* No reference counterpart: the kernel path serving one filter.
  *   SVPF_PATH_SINGLE  one workgroup holds the filter in LDS;

@@ -89,6 +89,7 @@ struct svpf_impl {
     bool coop_cooperative = false;
     bool captured_cooperative = false;
     int coop_spin_limit = 0;
+    int coop_fault_wg = -1;  // failure-path testing (svpf_set_coop_fault)
     // grid path (workspace allocated for 2 <= n <= SVPF_GRID_MAX_PARTICLES)
     int grid_mode = SVPF_GRID_AUTO;
     float* dgrid_ws = nullptr;
@@ -168,6 +169,13 @@ bool choose_coop_shape(svpf_impl* I, int cs, int stride) {
     return true;
 }
 
+// Exchange epochs to skip after a timed-out cooperative launch of T ticks: more than
+// the launch could have used (per tick one exchange per Stein iteration and the final
+// one, plus the commit), so the retry meets no word the failed launch tagged.
+uint32_t epochs_after_timeout(const SVPFState* s, int T) {
+    return (uint32_t)(1u << 16) + (uint32_t)T * (uint32_t)(std::max(s->stein_max_steps, 0) + 4);
+}
+
 int set_err(SVPFState* s, int code) {
     if (s && s->impl) s->impl->last_error = code;
     return code;
@@ -230,6 +238,7 @@ void pack_params(const SVPFState* s, const SVPFParams* p, svpf_fparams* o) {
     if (s->use_two_factor) f |= SVPF_F_TWO_FACTOR;
     o->nu_state = s->nu_state;
     o->coop_spin_limit = s->impl->coop_spin_limit;
+    o->coop_fault_wg = s->impl->coop_fault_wg;
     o->rho_fast = s->rho_fast;
     o->sigma_fast = s->sigma_fast;
     o->rho_slow = s->rho_slow;
@@ -574,7 +583,7 @@ void step_impl(SVPFState* s, float y_t, float y_prev, const SVPFParams* p, float
         // a plain launch's exchange timed out; repeat the tick with a cooperative launch, on
         // fresh exchange epochs
         I->coop_cooperative = true;
-        I->epoch += 1u << 16;
+        I->epoch += epochs_after_timeout(s, 1);
         pack_fstate(s, y_prev, I->hs());
         svpf_graph_invalidate(s);
         e = enqueue_tick(s, cs);
@@ -632,7 +641,7 @@ int run_device(SVPFState* s, const float* d_obs, int T, const SVPFParams* p, flo
         // cooperative launch, on fresh exchange epochs
         if (e != hipSuccess || I->hs()->status != 2 || !coop || I->coop_cooperative) break;
         I->coop_cooperative = true;
-        I->epoch += 1u << 16;
+        I->epoch += epochs_after_timeout(s, T);
         pack_fstate(s, y_prev0, I->hs());
         svpf_graph_invalidate(s);
     }
@@ -680,7 +689,7 @@ SVPFState* svpf_create(int n_particles, int n_stein_steps, float nu, hipStream_t
     if (!grid_only) ok = ok && hipMalloc(&I->d2f, 6 * (size_t)n_particles * sizeof(float)) == hipSuccess;
     // exchange words of the cooperative path: [2][4][n] records, [2][n] final positions,
     // [2][G][8] partials with G <= n / 8 + 1
-    const size_t xwords = 10 * (size_t)n_particles + 16 * ((size_t)n_particles / 8 + 2);
+    const size_t xwords = svpf_coop_xbuf_words_host(n_particles);
     ok = ok && hipMalloc(&I->xbuf, xwords * sizeof(uint64_t)) == hipSuccess;
     if (ok) {
         std::memset(I->hblock, 0, kBlockBytes);
@@ -1223,6 +1232,13 @@ int svpf_set_coop_spin_limit(SVPFState* s, int polls) {
     if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
     if (polls < -(1 << 20) || polls > (1 << 24)) return set_err(s, SVPF_RC_INVALID_ARGUMENT);
     s->impl->coop_spin_limit = polls;
+    return set_err(s, SVPF_RC_OK);
+}
+
+int svpf_set_coop_fault(SVPFState* s, int workgroup) {
+    if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
+    if (workgroup < -1 || workgroup >= s->impl->coop_G) return set_err(s, SVPF_RC_INVALID_ARGUMENT);
+    s->impl->coop_fault_wg = workgroup;
     return set_err(s, SVPF_RC_OK);
 }
 

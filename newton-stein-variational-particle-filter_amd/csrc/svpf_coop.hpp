@@ -30,10 +30,19 @@ struct CoopIO {
     float* h;
     float* hp;
     float* v;
-    uint64_t* xbuf;   // [2][3][n] transport records, then [2][n] final positions, then [2][G][PS_N] partials
+    uint64_t* xbuf;   // [2][4][n] transport records, [2][n] final positions, [2][G][PS_N] partials, then the
+                      // commit area (svpf_coop_xbuf_words)
     int G;            // cooperating workgroups
     int g;            // this workgroup
 };
+
+// Exchange buffer of one filter of n particles, in 64-bit words: the records,
+// final positions and partials (G <= n / 8 + 2), then the commit stage -- each
+// row's RMSProp state and log-weight, 2n floats -- and one commit counter.
+__host__ __device__ inline size_t svpf_coop_stage_offset(int n) {
+    return 10 * (size_t)n + 2 * (size_t)PS_N * ((size_t)n / 8 + 2);
+}
+__host__ __device__ inline size_t svpf_coop_xbuf_words(int n) { return svpf_coop_stage_offset(n) + (size_t)n + 1; }
 
 SVPF_DEV uint64_t tag_word(float v, uint32_t epoch) { return ((uint64_t)epoch << 32) | (uint64_t)__float_as_uint(v); }
 
@@ -382,6 +391,12 @@ struct CoopEngine {
             // the previous final positions become this tick's h_prev
             for (int i = threadIdx.x; i < n; i += BS) hpp[i] = hfin[i];
             __syncthreads();
+            // failure-path testing: one workgroup's waits of the launch's final exchange give
+            // up after 4 polls, after it has published its own rows
+            if (g == P.coop_fault_wg && t == t1 - 1) {
+                epoch_xor = 0x80000000u;
+                spin_limit = 4;
+            }
             float* hf = hfin;
             if (!gather(xf, n, ep, [hf](int k, float val) { hf[k] = val; })) return false;
             SVPF_TRACE_POINT(t, 41);
@@ -406,17 +421,43 @@ struct CoopEngine {
             SVPF_TRACE_POINT(t, 42);
         }
 
-        // ---- write back (own rows; workgroup 0 the scalars) ----
+        // ---- commit: all or nothing.  Each workgroup stages its rows' RMSProp state and
+        //      log-weight and arrives at the commit counter; the one arriving last (every
+        //      other workgroup has passed every exchange and staged) writes the filter's
+        //      particles, which every workgroup holds after the final exchange, the staged
+        //      rows and the scalars.  A launch in which any wait timed out never completes
+        //      the count and leaves the state as it was. ----
+        float* stage = reinterpret_cast<float*>(io.xbuf + svpf_coop_stage_offset(n));
+        uint64_t* commit = io.xbuf + svpf_coop_stage_offset(n) + n;
+        __shared__ int last_wg;
         if (cs == 0 && valid()) {
-            int i = row();
-            io.h[i] = hfin[i];
-            io.hp[i] = hpp[i];
-            io.v[i] = v;
-            if (D.log_w) D.log_w[i] = lw;
+            __hip_atomic_store(stage + row(), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(stage + n + row(), lw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (g == 0 && threadIdx.x == 0) {
-            fs.epoch = epoch - 1u;
-            *io.S = fs;
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const uint32_t ep_commit = epoch;  // one past the launch's last exchange
+            uint64_t old = __hip_atomic_load(commit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), nw;
+            do {
+                nw = ((uint32_t)(old >> 32) == ep_commit) ? old + 1u : (((uint64_t)ep_commit << 32) | 1u);
+            } while (!__hip_atomic_compare_exchange_weak(commit, &old, nw, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_AGENT));
+            last_wg = ((uint32_t)nw == (uint32_t)G) ? 1 : 0;
+        }
+        __syncthreads();
+        if (last_wg) {
+            __threadfence();
+            for (int i = threadIdx.x; i < n; i += BS) {
+                io.h[i] = hfin[i];
+                io.hp[i] = hpp[i];
+                io.v[i] = __hip_atomic_load(stage + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (D.log_w) D.log_w[i] = __hip_atomic_load(stage + n + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (threadIdx.x == 0) {
+                fs.epoch = epoch;
+                *io.S = fs;
+            }
         }
         return true;
     }

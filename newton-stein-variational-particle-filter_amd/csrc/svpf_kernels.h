@@ -46,6 +46,8 @@ extern "C" {
 // Cooperative latency kernel for one filter (256 threads per workgroup, CS lanes per row).
 int svpf_coop_rows(int cs);
 size_t svpf_coop_lds_bytes(int n);
+// 64-bit words of a filter's cooperative exchange buffer (records, positions, partials, commit stage).
+size_t svpf_coop_xbuf_words_host(int n);
 hipError_t svpf_launch_coop(int cs, int G, int stride, const svpf_fparams* params, svpf_fstate* fstate,
                             const svpf_fdesc* desc, float* pool_h, float* pool_hp, float* pool_v, uint64_t* xbuf,
                             int n, int t0, int t1, int cooperative, hipStream_t stream);

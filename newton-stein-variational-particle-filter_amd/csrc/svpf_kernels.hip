@@ -132,7 +132,9 @@ __global__ __launch_bounds__(BS) void svpf_coop_kernel(const svpf_fparams* __res
     CoopIO io{P, fstate, D, pool_h + D->off, pool_hp + D->off, pool_v + D->off, xbuf, G, g};
     CoopEngine<BS, CS> e;
     e.bind(lds, P->n, G, g);
-    if (!e.run(io, t0, t1) && threadIdx.x == 0) fstate->status = 2;
+    // a timed-out wait: the status word is set atomically and never written by the commit,
+    // which no launch with a timed-out wait reaches
+    if (!e.run(io, t0, t1) && threadIdx.x == 0) atomicOr(&fstate->status, 2);
 }
 
 #define SVPF_COOP_CASE(CSV)                                                                                  \
@@ -151,6 +153,7 @@ extern "C" hipError_t svpf_trace_dump(unsigned long long* out) {
 #endif
 
 extern "C" size_t svpf_coop_lds_bytes(int n) { return CoopEngine<256, 1>::lds_bytes(n); }
+extern "C" size_t svpf_coop_xbuf_words_host(int n) { return svpf_coop_xbuf_words(n); }
 
 extern "C" hipError_t svpf_launch_coop(int cs, int G, int stride, const svpf_fparams* params, svpf_fstate* fstate,
                                        const svpf_fdesc* desc, float* pool_h, float* pool_hp, float* pool_v,

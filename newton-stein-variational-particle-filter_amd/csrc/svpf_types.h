@@ -70,7 +70,9 @@ typedef struct svpf_fparams {
     float rho_fast, sigma_fast, rho_slow, sigma_slow;
     float bw_floor_fast, bw_floor_slow, guide_strength;
     int32_t n_stein;          // two-factor step: Stein iterations per annealing stage
-    float pad1[12];
+    int32_t coop_fault_wg;    // failure-path testing: this cooperative workgroup's final-exchange waits
+                              // give up (-1: none)
+    float pad1[11];
 } svpf_fparams;
 
 // Per-filter scalar state carried between ticks (host fields of SVPFState and

@@ -6,8 +6,10 @@ is no CPU fallback -- loading fails loudly when the library is missing.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
+import sys
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(_PKG_DIR, "libsvpf_amd.so")
@@ -104,7 +106,7 @@ EXPORTED = (
     "svpf_tile_occupancy_query", "svpf_diff2d_tape_floats", "svpf_diff2d_forward", "svpf_diff2d_backward",
     "svpf_set_grid_path", "svpf_get_path", "svpf_fully_fused_step_v2", "svpf_fully_fused_v2_supported",
     "svpf_coop_launch_is_cooperative", "svpf_set_coop_spin_limit", "svpf_digamma_f32",
-    "svpf_get_two_factor_particles", "svpf_run_sequence_device_noalloc",
+    "svpf_get_two_factor_particles", "svpf_run_sequence_device_noalloc", "svpf_set_coop_fault",
     # score diagnostics (include/svpf.h)
     "svpf_gradient_diagnostic_create", "svpf_gradient_diagnostic_destroy", "svpf_compute_nu_diagnostic",
     "svpf_compute_nu_diagnostic_simple", "svpf_compute_sigma_diagnostic_simple", "svpf_snapshot_particles",
@@ -136,6 +138,23 @@ V2_ARGS = (
 )
 
 _lib = None
+_exiting = False
+
+
+def _mark_exiting():
+    global _exiting
+    _exiting = True
+
+
+atexit.register(_mark_exiting)
+
+
+def finalizing() -> bool:
+    """True once the interpreter is shutting down.  Destructors then skip the
+    C-ABI destroy calls: the HIP runtime (and a profiler's tool library) may
+    already be finalising, and the process's device memory goes with it.
+    Explicit close() calls before that point free everything as usual."""
+    return _exiting or sys.is_finalizing()
 
 
 def load(path: str | None = None):
@@ -214,6 +233,8 @@ def load(path: str | None = None):
     L.svpf_get_path.argtypes = [SP]
     L.svpf_coop_launch_is_cooperative.restype = c_int
     L.svpf_coop_launch_is_cooperative.argtypes = [SP]
+    L.svpf_set_coop_fault.restype = c_int
+    L.svpf_set_coop_fault.argtypes = [SP, c_int]
     L.svpf_set_coop_spin_limit.restype = c_int
     L.svpf_set_coop_spin_limit.argtypes = [SP, c_int]
     L.svpf_get_coop_shape.argtypes = [SP, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]

@@ -33,7 +33,7 @@ FPARAMS_DTYPE = np.dtype([("n", "<i4"), ("flags", "<u4"), ("n_anneal", "<i4"), (
                          + [(f, "<f4") for f in _PARAM_FLOATS] + [("nu_state", "<f4"), ("coop_spin_limit", "<i4")]
                          + [(f, "<f4") for f in ("rho_fast", "sigma_fast", "rho_slow", "sigma_slow", "bw_floor_fast",
                                                  "bw_floor_slow", "guide_strength")]
-                         + [("n_stein", "<i4"), ("pad1", "<f4", (12,))])
+                         + [("n_stein", "<i4"), ("coop_fault_wg", "<i4"), ("pad1", "<f4", (11,))])
 FSTATE_DTYPE = np.dtype([("timestep", "<i4"), ("guide_initialized", "<i4"), ("stein_steps_used", "<i4"),
                          ("status", "<i4")]
                         + [(f, "<f4") for f in ("ksd_prev", "vol_prev", "h_mean_prev", "guide_mean", "guide_var",
@@ -116,6 +116,7 @@ def pack_params(cfg: dict) -> np.ndarray:
         f |= F_PAPER_SIGN
     p["n"] = cfg["n"]
     p["flags"] = f
+    p["coop_fault_wg"] = -1
     p["nu_state"] = max(float(cfg["nu_state"]), 2.5)
     p["n_anneal"] = cfg["n_anneal_steps"]
     p["stein_min"] = cfg["stein_min_steps"]

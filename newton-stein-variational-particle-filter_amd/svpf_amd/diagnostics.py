@@ -154,6 +154,8 @@ class Scores:
             self.g = None
 
     def __del__(self):
+        if _lib.finalizing():
+            return
         try:
             self.close()
         except Exception:

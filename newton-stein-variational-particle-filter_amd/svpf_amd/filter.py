@@ -86,6 +86,8 @@ class Filter:
             self.sp = None
 
     def __del__(self):
+        if _lib.finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -104,6 +104,8 @@ class TwoScale:
             self.ms = None
 
     def __del__(self):
+        if _lib.finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -23,6 +23,8 @@ import numpy as np
 from . import _lib
 from .config import student_t_const
 
+__all__ = ["SVPF", "SVPFResult"]
+
 
 @dataclass(frozen=True)
 class SVPFResult:
@@ -52,7 +54,7 @@ class SVPF:
 
     def __del__(self):
         sp = getattr(self, "_sp", None)
-        if sp:
+        if sp and not _lib.finalizing():
             self._L.svpf_destroy(sp)
             self._sp = None
 

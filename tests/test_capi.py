@@ -157,3 +157,36 @@ def test_digamma_is_bounded_and_accurate():
         assert L.svpf_digamma_f32(x) == pytest.approx(scipy.special.digamma(x), rel=3e-6, abs=3e-6), x
     for x in (0.0, -1.0, -0.5, -3e7, float("-inf"), float("inf"), float("nan")):
         assert math.isnan(L.svpf_digamma_f32(x)), x
+
+
+def test_import_pysvpf_by_the_reference_module_name():
+    import pysvpf
+    from svpf_amd import pysvpf as impl
+
+    assert pysvpf.SVPF is impl.SVPF and pysvpf.SVPFResult is impl.SVPFResult
+    for name in ("initialize", "step", "run", "reset", "set_params", "set_mim", "set_newton", "get_particles",
+                 "get_ess", "invalidate_graph", "is_graph_captured"):
+        assert hasattr(pysvpf.SVPF, name), name
+
+
+def test_destructors_skip_destroy_during_interpreter_shutdown(monkeypatch):
+    """During shutdown the HIP runtime may already be finalising: destructors leave
+    the state to the process exit instead of calling svpf_destroy; explicit close()
+    and destructors before shutdown destroy as usual."""
+    from svpf_amd import filter as flt
+
+    calls = []
+
+    class FakeLib:
+        def svpf_destroy(self, sp):
+            calls.append(sp)
+
+    f = flt.Filter.__new__(flt.Filter)
+    f.L, f.sp = FakeLib(), 1234
+    monkeypatch.setattr(_lib, "_exiting", True)
+    assert _lib.finalizing()
+    f.__del__()
+    assert calls == [] and f.sp == 1234
+    monkeypatch.setattr(_lib, "_exiting", False)
+    f.__del__()
+    assert calls == [1234] and f.sp is None

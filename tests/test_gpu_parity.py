@@ -356,3 +356,54 @@ def test_smoothing_matches_oracle(torch_cuda, lag, out_lag, adaptive_mu):
         assert abs(hm - want) <= 1e-4, (t, hm, want)
         yp = float(y[t])
     f.close()
+
+
+@pytest.mark.parametrize("wg", [0, 7, 15])
+def test_coop_partial_timeout_commits_nothing(torch_cuda, wg):
+    """One workgroup's waits at the final exchange give up while every other workgroup
+    completes (host-set fault, every kernel exits normally).  The launch must commit
+    nothing -- particles, previous particles, RMSProp state and scalars stay as they
+    were -- and report the time-out; with the fault cleared, the filter's next tick
+    equals that of a filter that never failed.  Also for a 6-tick device loop whose
+    last tick fails."""
+    from svpf_amd import _lib
+
+    torch = torch_cuda
+    n = 512
+    cfg = config.plain_config(n=n)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=51)
+    noise = data.predraw_noise(6, n, 5, seed=52)
+    y = data.sv_series(6, seed=53)[0]
+
+    def make():
+        f = Filter(cfg, path="coop")
+        f.set_particles(h0)
+        f.set_noise(dev(torch, noise["z_pred"]), dev(torch, noise["xi"]))
+        assert f.path() == "coop" and f.coop_shape()[1] == 16
+        return f
+
+    f = make()
+    L = f.L
+    assert L.svpf_set_coop_fault(f.sp, wg) == _lib.SVPF_RC_OK
+    print(f"coop fault in workgroup {wg}", flush=True)
+    with pytest.raises(RuntimeError, match="time"):
+        f.step(float(y[0]), 0.0)
+    assert L.svpf_last_error(f.sp) == _lib.SVPF_RC_TIMEOUT
+    assert np.array_equal(f.particles(), h0.astype(np.float32))
+    assert np.array_equal(d2h(f.state.h_prev, n), h0.astype(np.float32))
+    assert np.array_equal(f.rmsprop_state(), np.zeros(n, np.float32))
+    assert f.state.timestep == 0
+    # the device loop: the last tick's final exchange fails, nothing of the 6 ticks is committed
+    with pytest.raises(RuntimeError, match="time"):
+        f.run_device(dev(torch, y))
+    assert np.array_equal(f.particles(), h0.astype(np.float32)) and f.state.timestep == 0
+    assert L.svpf_set_coop_fault(f.sp, -1) == _lib.SVPF_RC_OK
+    f.step(float(y[0]), 0.0)
+    g = make()
+    g.step(float(y[0]), 0.0)
+    assert np.array_equal(f.particles(), g.particles())
+    assert np.array_equal(f.rmsprop_state(), g.rmsprop_state())
+    assert np.array_equal(f.last_outputs(), g.last_outputs())
+    assert L.svpf_set_coop_fault(f.sp, 16) == _lib.SVPF_RC_INVALID_ARGUMENT
+    f.close()
+    g.close()
