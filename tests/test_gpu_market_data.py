@@ -32,3 +32,26 @@ def test_real_series_matches_oracle(torch_cuda, name, n, path):
     assert f.path() == path
     o, out = run_oracle(cfg, y, noise, h0)
     compare(rec, out, tol_ll=1e-3)
+
+
+@pytest.mark.parametrize("name", ["spy_2008_crisis", "spy_2020_covid"])
+@pytest.mark.parametrize("n,path", [(200, "single"), (512, "coop"), (1024, "grid")])
+def test_real_series_full_feature_set_matches_oracle(torch_cuda, name, n, path):
+    """The production feature set (guided MIM predict with asymmetric rho, EKF guide with
+    adaptive strength, adaptive mu and sigma, full Newton, rejuvenation; the configuration
+    the reference's pysvpf set_defaults and two-scale filter use) on the real series, every
+    path against the FP64 oracle.  Same bars as the plain case; the FP64 oracle moves h_mean
+    by < 4e-7 when its start is perturbed by 1e-7 relative on these series and settings."""
+    from test_gpu_parity import FEATURE_CONFIGS
+
+    y = data.load_returns_binary(os.path.join(FIX, name + ".bin"))
+    mu = float(2 * np.log(np.std(y.astype(np.float64))))
+    cfg = config.make_config(n=n, mu=mu, rho=0.98, sigma_z=0.15, gamma=-0.2, **FEATURE_CONFIGS["production"])
+    noise = data.predraw_noise(len(y), n, cfg["stein_max_steps"], seed=21, mim=True, rejuv=True)
+    h0 = data.initial_particles(n, mu, 0.98, 0.15, seed=22)
+    print(f"market data {name} production n={n} path={path} T={len(y)}", flush=True)
+    f, rec = run_gpu(torch_cuda, cfg, y, noise, h0, path=path)
+    assert f.path() == path
+    o, out = run_oracle(cfg, y, noise, h0)
+    compare(rec, out, tol_ll=2e-3, tol_vol=2e-4)
+    f.close()
