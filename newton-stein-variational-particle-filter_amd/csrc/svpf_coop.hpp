@@ -38,11 +38,11 @@ struct CoopIO {
 
 // Exchange buffer of one filter of n particles, in 64-bit words: the records,
 // final positions and partials (G <= n / 8 + 2), then the commit stage -- each
-// row's RMSProp state and log-weight, 2n floats -- and one commit counter.
+// row's RMSProp state and log-weight as tagged words, 2n -- and one commit counter.
 __host__ __device__ inline size_t svpf_coop_stage_offset(int n) {
     return 10 * (size_t)n + 2 * (size_t)PS_N * ((size_t)n / 8 + 2);
 }
-__host__ __device__ inline size_t svpf_coop_xbuf_words(int n) { return svpf_coop_stage_offset(n) + (size_t)n + 1; }
+__host__ __device__ inline size_t svpf_coop_xbuf_words(int n) { return svpf_coop_stage_offset(n) + 2 * (size_t)n + 1; }
 
 SVPF_DEV uint64_t tag_word(float v, uint32_t epoch) { return ((uint64_t)epoch << 32) | (uint64_t)__float_as_uint(v); }
 
@@ -171,6 +171,13 @@ struct CoopEngine {
         s = 0.f;
         __syncthreads();
         uint32_t epoch = fs.epoch + 1u;  // epochs are never reused across launches
+        // the commit counter of this launch starts at 0 before workgroup 0 publishes anything
+        // (the previous launch on the stream has finished with it)
+        if (g == 0 && threadIdx.x == 0) {
+            __hip_atomic_store(io.xbuf + svpf_coop_stage_offset(n) + 2 * (size_t)n, (uint64_t)0, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_s_waitcnt(0);  // the clear has completed before anything is published
+        }
         uint64_t* xrec = io.xbuf;                          // [2][4][n]
         uint64_t* xfin = io.xbuf + 8 * (size_t)n;           // [2][n]
         uint64_t* xpart = xfin + 2 * (size_t)n;             // [2][G][PS_N]
@@ -422,37 +429,37 @@ struct CoopEngine {
         }
 
         // ---- commit: all or nothing.  Each workgroup stages its rows' RMSProp state and
-        //      log-weight and arrives at the commit counter; the one arriving last (every
-        //      other workgroup has passed every exchange and staged) writes the filter's
-        //      particles, which every workgroup holds after the final exchange, the staged
-        //      rows and the scalars.  A launch in which any wait timed out never completes
-        //      the count and leaves the state as it was. ----
-        float* stage = reinterpret_cast<float*>(io.xbuf + svpf_coop_stage_offset(n));
-        uint64_t* commit = io.xbuf + svpf_coop_stage_offset(n) + n;
+        //      log-weight as tagged words and arrives at the commit counter; the one
+        //      arriving last (every other workgroup has passed every exchange) gathers the
+        //      staged rows -- tagged, so no fence orders them -- and writes the filter's
+        //      particles, which every workgroup holds after the final exchange, those rows
+        //      and the scalars.  A launch in which any wait timed out never completes the
+        //      count, or fails the last gather, and leaves the state as it was. ----
+        uint64_t* stage = io.xbuf + svpf_coop_stage_offset(n);
+        uint64_t* commit = stage + 2 * (size_t)n;
+        const uint32_t ep_c = epoch;  // one past the launch's last exchange
         __shared__ int last_wg;
         if (cs == 0 && valid()) {
-            __hip_atomic_store(stage + row(), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(stage + n + row(), lw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            publish(stage + row(), v, ep_c);
+            publish(stage + n + row(), lw, ep_c);
         }
-        __threadfence();
         __syncthreads();
         if (threadIdx.x == 0) {
-            const uint32_t ep_commit = epoch;  // one past the launch's last exchange
-            uint64_t old = __hip_atomic_load(commit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), nw;
-            do {
-                nw = ((uint32_t)(old >> 32) == ep_commit) ? old + 1u : (((uint64_t)ep_commit << 32) | 1u);
-            } while (!__hip_atomic_compare_exchange_weak(commit, &old, nw, __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
-                                                         __HIP_MEMORY_SCOPE_AGENT));
-            last_wg = ((uint32_t)nw == (uint32_t)G) ? 1 : 0;
+            // workgroup 0 cleared the counter before its first publication, which every
+            // workgroup arriving here has gathered: one atomic add each, no retries
+            const uint64_t before = __hip_atomic_fetch_add(commit, (uint64_t)1, __ATOMIC_RELAXED,
+                                                           __HIP_MEMORY_SCOPE_AGENT);
+            last_wg = (before == (uint64_t)(G - 1)) ? 1 : 0;
         }
         __syncthreads();
         if (last_wg) {
-            __threadfence();
+            float* staged = reinterpret_cast<float*>(hsc);  // [2][n]; the records are dead now
+            if (!gather(stage, 2 * n, ep_c, [staged](int k, float val) { staged[k] = val; })) return false;
             for (int i = threadIdx.x; i < n; i += BS) {
                 io.h[i] = hfin[i];
                 io.hp[i] = hpp[i];
-                io.v[i] = __hip_atomic_load(stage + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (D.log_w) D.log_w[i] = __hip_atomic_load(stage + n + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                io.v[i] = staged[i];
+                if (D.log_w) D.log_w[i] = staged[n + i];
             }
             if (threadIdx.x == 0) {
                 fs.epoch = epoch;

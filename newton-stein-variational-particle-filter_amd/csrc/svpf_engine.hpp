@@ -301,6 +301,10 @@ struct Engine {
     //   acc += K (s_j + c_j rs d K)          = K s_j + c_j (h_i - h_j)/ell^2 K^2
     //   KSD: a1 += K s_j, a2 += d K^2, a3 += s_j d K^2, a4 += K^2 (4 d^2 K - 1)
     //   full Newton: hw += curv_j K + 2/ell^2 K^2 |3 d^2 - 1|, kn += K
+    // With d^2 K = 1 - K the two kernel-curvature factors are polynomials in K:
+    // 4 d^2 K - 1 = 3 - 4K and K^2 |3 d^2 - 1| = K |3 - 4K|.  On a KSD iteration
+    // the transport reuses the KSD sums, acc = a1 + sum_j c_j rs u_j with
+    // u = d K^2: 11 VALU operations per pair (one rcp) instead of 15.
     // ---------------------------------------------------------------------
     template <int MODE, bool KSD>
     SVPF_DEV void transport_sums(float ib, float rs, float (&acc)[R], float (&a1)[R], float (&a2)[R],
@@ -317,21 +321,28 @@ struct Engine {
             for (int r = 0; r < R; ++r) {
                 float d = ht[r] - e.x;
                 float K = __builtin_amdgcn_rcpf(fmaf(d, d, 1.0f));
-                float q = fmaf(e.z * d, K, e.y);
-                acc[r] = fmaf(K, q, acc[r]);
-                if (MODE == MODE_FULL) {
-                    float K2 = K * K;
-                    hw[r] = fmaf(e.w, K, hw[r]);
-                    hw[r] = fmaf(two_ib * K2, fabsf(fmaf(3.0f * d, d, -1.0f)), hw[r]);
-                    kn[r] += K;
-                }
                 if (KSD) {
                     float K2 = K * K;
                     float u = d * K2;
                     a1[r] = fmaf(K, e.y, a1[r]);
                     a2[r] += u;
                     a3[r] = fmaf(e.y, u, a3[r]);
-                    a4[r] = fmaf(K2, fmaf(4.0f * d * d, K, -1.0f), a4[r]);
+                    acc[r] = fmaf(e.z, u, acc[r]);  // repulsion; the attraction a1 joins after the loop
+                    float t = fmaf(-4.0f, K, 3.0f);
+                    a4[r] = fmaf(K2, t, a4[r]);
+                    if (MODE == MODE_FULL) {
+                        hw[r] = fmaf(e.w, K, hw[r]);
+                        hw[r] = fmaf(two_ib, K * fabsf(t), hw[r]);
+                        kn[r] += K;
+                    }
+                } else {
+                    float q = fmaf(e.z * d, K, e.y);
+                    acc[r] = fmaf(K, q, acc[r]);
+                    if (MODE == MODE_FULL) {
+                        hw[r] = fmaf(e.w, K, hw[r]);
+                        hw[r] = fmaf(two_ib, K * fabsf(fmaf(-4.0f, K, 3.0f)), hw[r]);
+                        kn[r] += K;
+                    }
                 }
             }
         };
@@ -358,6 +369,7 @@ struct Engine {
                 a2[r] = group_sum<CS>(a2[r]);
                 a3[r] = group_sum<CS>(a3[r]);
                 a4[r] = group_sum<CS>(a4[r]);
+                acc[r] += a1[r];
             }
         }
     }
