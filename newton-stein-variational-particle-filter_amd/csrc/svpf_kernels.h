@@ -118,6 +118,12 @@ hipError_t svpf_launch_2f(int bs, int tile, const svpf_fparams* P, svpf_fstate* 
                           float* h, int n, int t0, int t1, hipStream_t stream);
 hipError_t svpf_launch_2f_init(float* x2f, int n, float rho_f, float sigma_f, float rho_s, float sigma_s,
                                uint64_t seed, hipStream_t stream);
+// The joint state-parameter filter, one workgroup: mode 0 runs T steps at ticks
+// t0 .. t0 + T - 1 on y[0 .. T-1] (on y1 when y is null and T = 1); mode 1 draws
+// the initial particles.
+hipError_t svpf_launch_joint(const svpf_jparams* P, float* xs, float* gl, float* summ, const float* y, float y1,
+                             float* rec, int t0, int T, int mode, hipStream_t stream);
+size_t svpf_joint_lds_bytes_host(int n);
 #ifdef __cplusplus
 }
 #endif

@@ -126,9 +126,29 @@ typedef struct svpf_fdesc {
 // SVPF_PURPOSE_PREDICT, the slow factor's normal from SVPF_PURPOSE_PREDICT_SLOW;
 // the Langevin normals of iteration k from SVPF_PURPOSE_SVLD0 + k (fast) and
 // SVPF_PURPOSE_SVLD_SLOW0 + k (slow); its initial draw from SVPF_PURPOSE_INIT_2F.
+// Joint state-parameter filter: the diffusion normals of mu~ and rho~ from
+// SVPF_PURPOSE_JOINT_A, of sigma~ and h from SVPF_PURPOSE_JOINT_B; its initial
+// draw of (h, mu~) from SVPF_PURPOSE_JOINT_INIT_A and of (rho~, sigma~) from
+// SVPF_PURPOSE_JOINT_INIT_B.
 enum { SVPF_PURPOSE_PREDICT = 0, SVPF_PURPOSE_REJUV = 1, SVPF_PURPOSE_PREDICT_SLOW = 2, SVPF_PURPOSE_INIT = 255,
        SVPF_PURPOSE_INIT_2F = 254, SVPF_PURPOSE_SVLD0 = 16, SVPF_PURPOSE_SVLD_SLOW0 = 1024,
-       SVPF_PURPOSE_STATE_T0 = 128 };
+       SVPF_PURPOSE_STATE_T0 = 128, SVPF_PURPOSE_JOINT_A = 3, SVPF_PURPOSE_JOINT_B = 4,
+       SVPF_PURPOSE_JOINT_INIT_A = 253, SVPF_PURPOSE_JOINT_INIT_B = 252 };
+
+// Joint state-parameter filter: the step kernel's parameters (passed by value).
+typedef struct svpf_jparams {
+    int32_t n, n_stein;
+    float step_h, step_mu, step_rho, step_sigma;
+    float diff_mu, diff_rho, diff_sigma;
+    float thr_mu, thr_rho, thr_sigma;
+    float mu_pm, mu_pv, rho_pm, rho_pv, sig_pm, sig_pv;
+    float nu, st_const, lik_offset, prior_weight;
+    uint32_t k0, k1;
+    int32_t filter, pad;
+} svpf_jparams;
+// Words of one joint-filter step record: the reference's diagnostics structure
+// (11 floats, 3 ints, 2 floats) followed by the four bandwidths.
+#define SVPF_JOINT_RECW_DEV 20
 
 // ---------------------------------------------------------------------------
 // Differentiable form: B filters of n particles with the plain feature set

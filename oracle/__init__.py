@@ -446,3 +446,78 @@ class TwoFactorOracle:
 
     def ekf(self, y_t):
         _lib2f().svpfo_ekf_2f_update(ctypes.byref(self.c), ctypes.byref(self.st), float(y_t))
+
+
+# ---- the joint state-parameter filter (svpf_joint/) ----
+JOINT_INT_FIELDS = ("n", "n_stein_steps")
+JOINT_DOUBLE_FIELDS = (
+    "step_h", "step_mu", "step_rho", "step_sigma", "diffusion_mu", "diffusion_rho", "diffusion_sigma",
+    "collapse_thresh_mu", "collapse_thresh_rho", "collapse_thresh_sigma", "mu_prior_mean", "mu_prior_var",
+    "rho_prior_mean", "rho_prior_var", "sigma_prior_mean", "sigma_prior_var", "nu", "student_t_const",
+    "student_t_implied_offset", "lik_offset", "prior_weight",
+)
+JOINT_OUT_DOUBLE = ("h_mean", "vol_mean", "mu_mean", "rho_mean", "sigma_mean", "mu_std", "rho_std", "sigma_std",
+                    "std_mu_tilde", "std_rho_tilde", "std_sigma_tilde", "ess", "log_likelihood")
+
+
+class JointCfg(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_int) for f in JOINT_INT_FIELDS] + [(f, ctypes.c_double) for f in JOINT_DOUBLE_FIELDS]
+
+
+class JointOut(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_double) for f in JOINT_OUT_DOUBLE] + [("bw", ctypes.c_double * 4)] + [
+        (f, ctypes.c_int) for f in ("mu_collapsed", "rho_collapsed", "sigma_collapsed", "pad_")]
+
+
+def _libj():
+    L = lib()
+    P = ctypes.c_void_p
+    L.svpfo_joint_init.restype = None
+    L.svpfo_joint_init.argtypes = [P, ctypes.c_uint64, P, P]
+    L.svpfo_joint_step.restype = ctypes.c_int
+    L.svpfo_joint_step.argtypes = [P, P, P, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint32, P]
+    L.svpfo_joint_grad.restype = None
+    L.svpfo_joint_grad.argtypes = [P, P, ctypes.c_double, ctypes.c_double, P, P]
+    return L
+
+
+def make_cfg_joint(cfg: dict) -> JointCfg:
+    c = JointCfg()
+    for f in JOINT_INT_FIELDS:
+        setattr(c, f, int(cfg[f]))
+    for f in JOINT_DOUBLE_FIELDS:
+        setattr(c, f, float(cfg[f]))
+    return c
+
+
+class JointOracle:
+    """The joint state-parameter filter run by the oracle; x is [4, n] = h, mu~, rho~, sigma~ (float64)."""
+
+    def __init__(self, cfg: dict, x0=None, hp0=None, seed=0):
+        self.cfg = dict(cfg)
+        self.c = make_cfg_joint(cfg)
+        n = int(cfg["n"])
+        if x0 is None:
+            self.x, self.hp = np.zeros((4, n)), np.zeros(n)
+            _libj().svpfo_joint_init(ctypes.byref(self.c), seed, _ptr(self.x), _ptr(self.hp))
+        else:
+            self.x = np.array(x0, dtype=np.float64).reshape(4, n).copy()
+            self.hp = (self.x[0] if hp0 is None else np.asarray(hp0, dtype=np.float64)).reshape(n).copy()
+
+    def step(self, y_t, seed, tick):
+        o = JointOut()
+        rc = _libj().svpfo_joint_step(ctypes.byref(self.c), _ptr(self.x), _ptr(self.hp), float(y_t), seed, tick,
+                                      ctypes.byref(o))
+        if rc:
+            raise RuntimeError("oracle joint step failed")
+        d = {f: getattr(o, f) for f in JOINT_OUT_DOUBLE}
+        d["bw"] = list(o.bw)
+        for f in ("mu_collapsed", "rho_collapsed", "sigma_collapsed"):
+            d[f] = getattr(o, f)
+        return d
+
+    def grad(self, xj, h_prev, y_t):
+        g, lw = np.zeros(4), np.zeros(1)
+        xa = np.array(xj, dtype=np.float64).reshape(4)
+        _libj().svpfo_joint_grad(ctypes.byref(self.c), _ptr(xa), float(h_prev), float(y_t), _ptr(g), _ptr(lw))
+        return g, float(lw[0])

@@ -70,7 +70,8 @@ double svpfo_normal_from_bits(uint32_t a, uint32_t b) {
 
 /* Noise purposes (third counter word). */
 enum { PURPOSE_PREDICT = 0, PURPOSE_REJUV = 1, PURPOSE_PREDICT_SLOW = 2, PURPOSE_INIT = 255, PURPOSE_INIT_2F = 254,
-       PURPOSE_SVLD0 = 16, PURPOSE_SVLD_SLOW0 = 1024, PURPOSE_STATE_T0 = 128 };
+       PURPOSE_SVLD0 = 16, PURPOSE_SVLD_SLOW0 = 1024, PURPOSE_STATE_T0 = 128, PURPOSE_JOINT_A = 3,
+       PURPOSE_JOINT_B = 4, PURPOSE_JOINT_INIT_A = 253, PURPOSE_JOINT_INIT_B = 252 };
 
 static void philox_draw(uint64_t seed, uint32_t filter, uint32_t tick, uint32_t purpose, uint32_t i,
                         uint32_t out[4]) {
@@ -1187,5 +1188,183 @@ int svpfo_tick_2f(const svpfo_2f_cfg* c, svpfo_2f_state* st, double* hf, double*
     st->h_mean_prev = out->h_mean;
     st->timestep++;
     free(lw); free(gf); free(gs); free(xn);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Joint state-parameter filter (svpf_joint/)                               */
+/* ------------------------------------------------------------------------ */
+static double sigm(double x) { return 1.0 / (1.0 + exp(-x)); }
+/* svpf_joint_kernels.cu:28-30: exp with its argument clamped to [-20, 20] */
+static double jsafe_exp(double x) { return exp(clamp_d(x, -20.0, 20.0)); }
+
+/* svpf_joint.cu:138-171, svpf_joint_kernels.cu:47-73 */
+void svpfo_joint_init(const svpfo_joint_cfg* c, uint64_t seed, double* x, double* hp) {
+    const int n = c->n;
+    for (int i = 0; i < n; ++i) {
+        uint32_t a[4], b[4];
+        philox_draw(seed, 0u, 0u, PURPOSE_JOINT_INIT_A, (uint32_t)i, a);
+        philox_draw(seed, 0u, 0u, PURPOSE_JOINT_INIT_B, (uint32_t)i, b);
+        x[i] = c->mu_prior_mean + 0.5 * svpfo_normal_from_bits(a[0], a[1]);
+        x[n + i] = c->mu_prior_mean + 0.2 * svpfo_normal_from_bits(a[2], a[3]);
+        x[2 * n + i] = c->rho_prior_mean + 0.2 * svpfo_normal_from_bits(b[0], b[1]);
+        x[3 * n + i] = c->sigma_prior_mean + 0.2 * svpfo_normal_from_bits(b[2], b[3]);
+        hp[i] = x[i];
+    }
+}
+
+/* svpf_joint_kernels.cu:134-320 */
+void svpfo_joint_grad(const svpfo_joint_cfg* c, const double xj[4], double h_prev, double y_t, double g[4],
+                      double* log_w) {
+    const double h = xj[0], mt = xj[1], rt = xj[2], st = xj[3];
+    const double mu = mt, rho = sigm(rt);
+    double sigma = jsafe_exp(st);
+    if (sigma < 0.01) sigma = 0.01;
+    const double var = sigma * sigma;
+    const double ysq = y_t * y_t;
+    const double A = ysq / (jsafe_exp(h) + 1e-8) / c->nu;
+    *log_w = c->student_t_const - 0.5 * h - (c->nu + 1.0) * 0.5 * log1p(A > -0.999 ? A : -0.999);
+    const double gh_lik = (log(ysq + 1e-10) - h + c->lik_offset) / 1.4;
+    /* observation surprise against the previous h */
+    const double z2o = ysq / (jsafe_exp(h_prev) + 1e-8);
+    const double surprise = z2o - 4.0 > 0.0 ? z2o - 4.0 : 0.0;
+    const double inflate = 1.0 + 0.3 * (surprise < 20.0 ? surprise : 20.0);
+    double vm = var * inflate;
+    if (vm < 0.03 * 0.03) vm = 0.03 * 0.03;
+    const int panic = z2o > 9.0;
+    if (panic && vm < 1.0) vm = 1.0;
+    const double tw = panic ? 0.3 : 1.0;
+    const double diff = h - (mu + rho * (h_prev - mu));
+    const double gh_tr = tw * (-diff / vm);
+    const double gm_tr = tw * (diff / vm) * (1.0 - rho);
+    const double gr_tr = tw * (diff / vm) * (h_prev - mu) * rho * (1.0 - rho);
+    const double gs_tr = diff * diff / var - 1.0;
+    double fobs = 0.0;
+    if (z2o > 9.0) fobs = (z2o - 9.0) * 0.5 < 5.0 ? (z2o - 9.0) * 0.5 : 5.0;
+    else if (z2o < 1.5) fobs = -(1.5 - z2o);
+    const double mr = -0.2 * (st - c->sigma_prior_mean);
+    const double pm = -(mt - c->mu_prior_mean) / c->mu_prior_var;
+    const double pr = -(rt - c->rho_prior_mean) / c->rho_prior_var;
+    const double ps = -(st - c->sigma_prior_mean) / c->sigma_prior_var;
+    g[0] = clamp_d(gh_lik + gh_tr, -10.0, 10.0);
+    g[1] = clamp_d(gm_tr + c->prior_weight * pm, -5.0, 5.0);
+    g[2] = clamp_d(gr_tr + c->prior_weight * pr, -5.0, 5.0);
+    g[3] = clamp_d(gs_tr + fobs + mr + c->prior_weight * ps, -5.0, panic ? 10.0 : 5.0);
+}
+
+int svpfo_joint_step(const svpfo_joint_cfg* c, double* x, double* hp, double y_t, uint64_t seed, uint32_t tick,
+                     svpfo_joint_out* out) {
+    const int n = c->n;
+    double* g = (double*)malloc((size_t)4 * n * sizeof(double));
+    double* xn = (double*)malloc((size_t)4 * n * sizeof(double));
+    double* lw = (double*)malloc((size_t)n * sizeof(double));
+    if (!g || !xn || !lw) return -1;
+    /* predict with parameter diffusion (svpf_joint_kernels.cu:79-128) */
+    for (int i = 0; i < n; ++i) {
+        uint32_t a[4], b[4];
+        philox_draw(seed, 0u, tick, PURPOSE_JOINT_A, (uint32_t)i, a);
+        philox_draw(seed, 0u, tick, PURPOSE_JOINT_B, (uint32_t)i, b);
+        const double h = x[i];
+        hp[i] = h;
+        const double mt = clamp_d(x[n + i] + c->diffusion_mu * svpfo_normal_from_bits(a[0], a[1]), -10.0, 5.0);
+        const double rt = clamp_d(x[2 * n + i] + c->diffusion_rho * svpfo_normal_from_bits(a[2], a[3]), -5.0, 5.0);
+        const double st = clamp_d(x[3 * n + i] + c->diffusion_sigma * svpfo_normal_from_bits(b[0], b[1]), -5.0, 2.0);
+        x[n + i] = mt;
+        x[2 * n + i] = rt;
+        x[3 * n + i] = st;
+        double sigma = jsafe_exp(st);
+        if (sigma < 0.01) sigma = 0.01;
+        const double rho = sigm(rt);
+        x[i] = clamp_logvol(mt + rho * (h - mt) + sigma * svpfo_normal_from_bits(b[2], b[3]));
+    }
+    /* bandwidths (svpf_joint_kernels.cu:620-663): sqrt(2 var / ln(n+1)), clamped */
+    double ib[4];
+    for (int d = 0; d < 4; ++d) {
+        double s1 = 0.0, s2 = 0.0;
+        for (int i = 0; i < n; ++i) { s1 += x[d * n + i]; s2 += x[d * n + i] * x[d * n + i]; }
+        const double m = s1 / n, var = s2 / n - m * m;
+        double bsq = 2.0 * var / log((double)n + 1.0);
+        if (bsq < 1e-6) bsq = 1e-6;
+        const double bw = clamp_d(sqrt(bsq), 0.01, 2.0);
+        out->bw[d] = bw;
+        ib[d] = 1.0 / (bw * bw + 1e-8);
+    }
+    /* With no Stein iteration the log weights are those of the predicted particles
+     * (the reference's diagnostics read a buffer no kernel has written then). */
+    if (c->n_stein_steps < 1)
+        for (int j = 0; j < n; ++j) {
+            double xj[4] = {x[j], x[n + j], x[2 * n + j], x[3 * n + j]};
+            svpfo_joint_grad(c, xj, hp[j], y_t, &g[4 * j], &lw[j]);
+        }
+    /* Stein iterations (svpf_joint.cu:207-276, svpf_joint_kernels.cu:326-470).  Every
+     * particle reads the positions of the previous iteration (Jacobi): the reference's
+     * kernel stages all positions in shared memory before its update, which is this
+     * reading when its blocks do not interleave. */
+    for (int s = 0; s < c->n_stein_steps; ++s) {
+        for (int j = 0; j < n; ++j) {
+            double xj[4] = {x[j], x[n + j], x[2 * n + j], x[3 * n + j]};
+            svpfo_joint_grad(c, xj, hp[j], y_t, &g[4 * j], &lw[j]);
+        }
+        for (int j = 0; j < n; ++j) {
+            const double zo = y_t * y_t / (exp(hp[j]) + 1e-8);
+            const int sur = zo > 9.0;
+            const double eff[4] = {c->step_h * (sur ? 5.0 : 1.0), c->step_mu * (sur ? 2.0 : 1.0), c->step_rho,
+                                   c->step_sigma * (sur ? 4.0 : 1.0)};
+            double phi[4] = {0.0, 0.0, 0.0, 0.0}, sk = 0.0;
+            for (int i = 0; i < n; ++i) {
+                double d[4], dist = 0.0;
+                for (int q = 0; q < 4; ++q) {
+                    d[q] = x[q * n + i] - x[q * n + j];
+                    dist += d[q] * d[q] * ib[q];
+                }
+                const double K = 1.0 / (1.0 + dist);
+                sk += K;
+                for (int q = 0; q < 4; ++q) phi[q] += K * g[4 * i + q] - 2.0 * d[q] * ib[q] * K * K;
+            }
+            double norm = sk < (double)n ? sk : (double)n;
+            if (norm < 32.0) norm = 32.0;
+            xn[j] = clamp_logvol(x[j] + eff[0] * phi[0] / norm);
+            xn[n + j] = clamp_d(x[n + j] + eff[1] * phi[1] / norm, -10.0, 5.0);
+            xn[2 * n + j] = clamp_d(x[2 * n + j] + eff[2] * phi[2] / norm, -5.0, 5.0);
+            xn[3 * n + j] = clamp_d(x[3 * n + j] + eff[3] * phi[3] / norm, -5.0, 2.0);
+        }
+        memcpy(x, xn, (size_t)4 * n * sizeof(double));
+    }
+    /* extract (svpf_joint_kernels.cu:476-614) and the host diagnostics (svpf_joint.cu:280-345) */
+    double sh = 0.0, sm = 0.0, sr = 0.0, ss = 0.0, qm = 0.0, qr = 0.0, qs = 0.0;
+    double tm = 0.0, tr = 0.0, ts = 0.0, um = 0.0, ur = 0.0, us = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double mt = x[n + i], rt = x[2 * n + i], st = x[3 * n + i];
+        const double r = sigm(rt), sg = exp(st);
+        sh += x[i];
+        sm += mt; sr += r; ss += sg;
+        qm += mt * mt; qr += r * r; qs += sg * sg;
+        tm += mt; tr += rt; ts += st;
+        um += mt * mt; ur += rt * rt; us += st * st;
+    }
+    const double inv = 1.0 / n;
+    out->h_mean = sh * inv;
+    out->vol_mean = exp(0.5 * out->h_mean);
+    out->mu_mean = sm * inv;
+    out->rho_mean = sr * inv;
+    out->sigma_mean = ss * inv;
+#define SDV(q, m) sqrt(((q) * inv - (m) * (m)) > 0.0 ? ((q) * inv - (m) * (m)) : 0.0)
+    out->mu_std = SDV(qm, out->mu_mean);
+    out->rho_std = SDV(qr, out->rho_mean);
+    out->sigma_std = SDV(qs, out->sigma_mean);
+    out->std_mu_tilde = SDV(um, tm * inv);
+    out->std_rho_tilde = SDV(ur, tr * inv);
+    out->std_sigma_tilde = SDV(us, ts * inv);
+#undef SDV
+    out->mu_collapsed = out->std_mu_tilde < c->collapse_thresh_mu;
+    out->rho_collapsed = out->std_rho_tilde < c->collapse_thresh_rho;
+    out->sigma_collapsed = out->std_sigma_tilde < c->collapse_thresh_sigma;
+    double mx = lw[0];
+    for (int i = 1; i < n; ++i) if (lw[i] > mx) mx = lw[i];
+    double sw = 0.0, sw2 = 0.0;
+    for (int i = 0; i < n; ++i) { double w = exp(lw[i] - mx); sw += w; sw2 += w * w; }
+    out->ess = sw * sw / (sw2 + 1e-10);
+    out->log_likelihood = mx + log(sw) - log((double)n);
+    free(g); free(xn); free(lw);
     return 0;
 }

@@ -237,6 +237,37 @@ int svpfo_tick_2f(const svpfo_2f_cfg* c, svpfo_2f_state* st, double* hf, double*
 void svpfo_init_2f(int n, double rho_fast, double sigma_fast, double rho_slow, double sigma_slow, uint64_t seed,
                    double* hf, double* hs);
 
+/* ---- the joint state-parameter filter (svpf_joint/src/svpf_joint.cu,
+ * svpf_joint/src/svpf_joint_kernels.cu): particles (h, mu~, rho~, sigma~) with
+ * mu = mu~, rho = sigmoid(rho~), sigma = exp(sigma~), moved by a 4-D IMQ Stein
+ * transport.  Configuration fields in the order of SVPFJointConfig
+ * (svpf_joint/include/svpf_joint.cuh). ---- */
+typedef struct svpfo_joint_cfg {
+    int n, n_stein_steps;
+    double step_h, step_mu, step_rho, step_sigma;
+    double diffusion_mu, diffusion_rho, diffusion_sigma;
+    double collapse_thresh_mu, collapse_thresh_rho, collapse_thresh_sigma;
+    double mu_prior_mean, mu_prior_var, rho_prior_mean, rho_prior_var, sigma_prior_mean, sigma_prior_var;
+    double nu, student_t_const, student_t_implied_offset, lik_offset, prior_weight;
+} svpfo_joint_cfg;
+/* Per-step outputs: SVPFJointDiagnostics plus the bandwidths. */
+typedef struct svpfo_joint_out {
+    double h_mean, vol_mean, mu_mean, rho_mean, sigma_mean, mu_std, rho_std, sigma_std;
+    double std_mu_tilde, std_rho_tilde, std_sigma_tilde, ess, log_likelihood;
+    double bw[4];
+    int mu_collapsed, rho_collapsed, sigma_collapsed, pad_;
+} svpfo_joint_out;
+/* x: [4][n] = h | mu~ | rho~ | sigma~, hp: [n] previous h.  Initial draw from the
+ * counter generator (purposes 253: h and mu~, 252: rho~ and sigma~; tick 0). */
+void svpfo_joint_init(const svpfo_joint_cfg* c, uint64_t seed, double* x, double* hp);
+/* One step at tick `tick`: predict normals from purposes 3 (mu~, rho~) and 4
+ * (sigma~, h) of the counter generator. */
+int svpfo_joint_step(const svpfo_joint_cfg* c, double* x, double* hp, double y_t, uint64_t seed, uint32_t tick,
+                     svpfo_joint_out* out);
+/* The per-particle score (svpf_joint_kernels.cu:134-320): g[4], log weight. */
+void svpfo_joint_grad(const svpfo_joint_cfg* c, const double xj[4], double h_prev, double y_t, double g[4],
+                      double* log_w);
+
 /* Counter-based generator: Philox4x32-10 (Salmon et al., SC'11). */
 void svpfo_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* Student-t state noise from the counter generator (particle i, tick, filter). */
