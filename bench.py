@@ -152,6 +152,53 @@ def time_single(torch, T=1000, warm=50):
     return us_device, float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
 
 
+JOINT_FLOP_PER_PAIR = 31  # 4 differences, 4 FMA distance, reciprocal, K^2, 8 FMA sums, mass
+
+
+def time_joint(torch, T=1000, warm=50):
+    """The joint state-parameter filter (include/svpf_joint.h) at the reference defaults
+    (512 particles, 5 Stein iterations): svpf_joint_step with diagnostics per tick (the
+    reference test's loop, p50 / p99), and the device loop over T ticks in one launch, at
+    N = 512 and 2048; the FP64 oracle on one thread as CPU baseline."""
+    import oracle
+    from svpf_amd import joint
+
+    y = np.random.default_rng(3).standard_normal(T + warm).astype(np.float32) * np.float32(np.exp(-1.75))
+    out = {"workload": "one joint filter, reference defaults (5 Stein iterations), synthetic N(0, e^-3.5) series"}
+    dy = torch.tensor(y, device="cuda")
+    for n in (512, 2048):
+        f = joint.JointFilter(seed=1, n_particles=n)
+        f.run_device(dy[:warm])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        f.run_device(dy[warm:])
+        torch.cuda.synchronize()
+        us = (time.perf_counter() - t0) / T * 1e6
+        flops = 5.0 * n * n * JOINT_FLOP_PER_PAIR
+        out[f"n{n}_us_per_step_device_loop"] = us
+        out[f"n{n}_pair_tflops"] = flops / (us * 1e-6) / 1e12
+        if n == 512:
+            lat = []
+            for t in range(200):
+                t1 = time.perf_counter()
+                f.step(float(y[t]))
+                lat.append(time.perf_counter() - t1)
+            lat = np.array(lat[10:]) * 1e6
+            out["n512_step_with_diag_us_p50"] = float(np.percentile(lat, 50))
+            out["n512_step_with_diag_us_p99"] = float(np.percentile(lat, 99))
+        f.close()
+    c = joint.default_config()
+    ocfg = {"n": 512, "n_stein_steps": 5}
+    ocfg.update({k: c[k] for k in joint.CONFIG_FIELDS})
+    o = oracle.JointOracle(ocfg, seed=1)
+    t0 = time.perf_counter()
+    for t in range(20):
+        o.step(float(y[t]), 1, t)
+    out["cpu_baseline"] = {"s_per_step_n512": (time.perf_counter() - t0) / 20, "cores": 1, "kind": "port",
+                           "sample": "FP64 C oracle (svpfo_joint_step), N=512, 20 steps"}
+    return out
+
+
 def time_fused_v2(torch, n=512, T=1000, warm=100):
     """The single-workgroup fused step (svpf_fully_fused_step_v2) with the
     reference driver's arguments (10 Stein iterations over 5 stages, full
@@ -296,6 +343,24 @@ def measured_traffic():
 def cpu_threads():
     """Host threads for the CPU baselines: the box's CPU share (at most 16)."""
     return max(1, min(16, os.cpu_count() or 1))
+
+
+def cpu_info():
+    """The host CPU the baselines ran on: model name, logical CPUs, CPUs this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = None
+    return {"cpu_model": model, "os_cpu_count": os.cpu_count(), "cpus_available": avail}
 
 
 def cpu_baseline(cfgs, ys, h0s, n, ticks=3):
@@ -515,6 +580,8 @@ def profile_only(torch, what, B, K, W):
         log(json.dumps(time_extended(torch, T=K)))
     elif what == "v2":
         log(json.dumps(time_fused_v2(torch, T=K, warm=W)))
+    elif what == "joint":
+        log(json.dumps(time_joint(torch, T=K, warm=W)))
     elif what == "diff":
         theta, dys, dh0 = diff_workload()
         fwd, bwd, _ = time_diff(torch, theta, dys, dh0, 32)
@@ -532,7 +599,7 @@ def main():
     ap.add_argument("--B", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
     ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "large", "v2",
-                                       "extended"],
+                                       "extended", "joint"],
                     default=None,
                     help="run one workload once and exit (for profiler runs): the N=2048 tile group of the "
                          "batched workload preceded by a 256 MiB device copy (traffic calibration), the "
@@ -588,6 +655,8 @@ def main():
         extras["extended_features"] = time_extended(torch)
         log("fused single-workgroup step (svpf_fully_fused_step_v2), N=512")
         extras["fused_v2"] = time_fused_v2(torch)
+        log("joint state-parameter filter, N=512 and 2048")
+        extras["joint"] = time_joint(torch)
         log("differentiable filter B=256 N=1024 T=500")
         theta, dys, dh0 = diff_workload()
         T_d = dys.shape[1]
@@ -673,6 +742,7 @@ def main():
                       "bucket {128..2048} x 8 ticks after 1 warm-up, weighted to the workload N mix",
             "s_per_step_by_n": {str(k): v for k, v in cpu_bucket.items()},
             "oracle_fp64_1thread_filter_steps_per_s": oracle_rate,
+            **cpu_info(),
         },
     }
     rec.update(extras)
