@@ -331,6 +331,10 @@ struct DiffEngine {
                 sc[0] = ell2;
                 sc[1] = __int_as_float(a);
                 sc[2] = __int_as_float(b);
+                if (D.mpair) {
+                    D.mpair[2 * t] = a;
+                    D.mpair[2 * t + 1] = b;
+                }
             }
         }
         const float bw = fminf(fmaxf(sqrtf(ell2), 0.01f), 2.0f);

@@ -237,6 +237,9 @@ struct D2Engine {
                 sc[0] = e2[0];
                 sc[1] = e2[1];
                 for (int q = 0; q < 4; ++q) sc[2 + q] = __int_as_float(pr[q]);
+                if (D.mpair) {
+                    for (int q = 0; q < 4; ++q) D.mpair[4 * t + q] = pr[q];
+                }
             }
         }
         const float rsf = 1.0f / fminf(fmaxf(sqrtf(e2[0]), 0.01f), 2.0f);

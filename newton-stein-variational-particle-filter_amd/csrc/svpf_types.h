@@ -190,6 +190,8 @@ typedef struct svpf_ddesc {
     int32_t pad;
     float* drift;         // [T][S][n] Stein drift phi of every iteration (forward output, optional): the
                           // backward's recompute replays the positions from it instead of redoing the transport
+    int32_t* mpair;       // [T][2] the median pair (a, b) the adjoint differentiates through, -1 where the
+                          // bandwidth is clamped (written where the tape is recorded; optional)
 } svpf_ddesc;
 
 // ---------------------------------------------------------------------------
@@ -234,6 +236,8 @@ typedef struct svpf_d2desc {
     int32_t pad;
     float* drift;         // [T][S][2][n] Stein drift of every iteration (forward output, optional): the
                           // backward's recompute replays the positions from it instead of redoing the transport
+    int32_t* mpair;       // [T][2][2] per coordinate the median pair (a, b) the adjoint differentiates
+                          // through, -1 where that bandwidth is clamped (optional)
 } svpf_d2desc;
 
 #ifdef __cplusplus

@@ -28,8 +28,8 @@ DCONFIG_DTYPE = np.dtype([("n", "<i4"), ("S", "<i4"), ("T", "<i4"), ("C", "<i4")
                           ("rms_rho", "<f4"), ("rms_eps", "<f4"), ("pad1", "<f4", (3,)), ("pad2", "<f4", (8,))])
 DDESC_DTYPE = np.dtype([(f, "<u8") for f in ("theta", "y", "h0", "z", "xi", "pred_ll", "ell2", "ckpt", "tape", "gpl",
                                               "gtheta", "gh0", "hT", "seed")]
-                       + [("filter_id", "<u4"), ("pad", "<i4"), ("drift", "<u8")])
-assert DCONFIG_DTYPE.itemsize == 96 and DDESC_DTYPE.itemsize == 128
+                       + [("filter_id", "<u4"), ("pad", "<i4"), ("drift", "<u8"), ("mpair", "<u8")])
+assert DCONFIG_DTYPE.itemsize == 96 and DDESC_DTYPE.itemsize == 136
 
 
 def _bind(L):
@@ -53,11 +53,14 @@ class DiffBatch:
     drift_budget_gb: the forward keeps every iteration's Stein drift for the
     backward (which then replays positions instead of recomputing the
     transport) when that buffer fits this many GB.
+    record_pairs: keep, per tick, the median pair (a, b) the adjoint
+    differentiates through (``self.mpair`` [B, T, 2], -1 where the bandwidth
+    is clamped), filled by the backward.
     """
 
     def __init__(self, y, h0, n_stein=5, checkpoint_every=32, noise=None, seeds=None, gamma=0.0, lik_offset=0.0,
                  temperature=1.0, rmsprop_rho=0.9, rmsprop_eps=1e-6, paper_sign=False, device="cuda",
-                 drift_budget_gb=32.0):
+                 drift_budget_gb=32.0, record_pairs=False):
         self.L = _lib.load()
         _bind(self.L)
         dev = torch.device(device)
@@ -84,6 +87,7 @@ class DiffBatch:
         # kept when it fits the budget; otherwise the backward recomputes the transport
         drift_gb = B * T * self.S * n * 4 / 1e9
         self.drift = torch.empty((B, T, self.S, n), **f32) if drift_gb <= drift_budget_gb else None
+        self.mpair = torch.full((B, T, 2), -1, dtype=torch.int32, device=dev) if record_pairs else None
         self.noise = None
         if noise is not None:
             self.noise = {k: torch.as_tensor(noise[k], **f32).contiguous() for k in ("z", "xi")}
@@ -115,6 +119,8 @@ class DiffBatch:
             d["hT"] = ptr(self.hT, b, n)
             if self.drift is not None:
                 d["drift"] = ptr(self.drift, b, T * self.S * n)
+            if self.mpair is not None:
+                d["mpair"] = ptr(self.mpair, b, 2 * T)
             d["seed"] = seeds[b]
             d["filter_id"] = b
         self.desc = torch.from_numpy(desc.view(np.uint8).reshape(-1).copy()).to(dev)

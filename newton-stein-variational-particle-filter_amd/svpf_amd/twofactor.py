@@ -28,8 +28,8 @@ D2CONFIG_DTYPE = np.dtype([("n", "<i4"), ("S", "<i4"), ("T", "<i4"), ("C", "<i4"
                            ("rms_eps", "<f4"), ("pad1", "<f4", (8,))])
 D2DESC_DTYPE = np.dtype([(f, "<u8") for f in ("theta", "y", "x0", "z", "xi", "out", "ell2", "ckpt", "tape", "gpl",
                                               "gtheta", "gx0", "xT", "seed")]
-                        + [("filter_id", "<u4"), ("pad", "<i4"), ("drift", "<u8")])
-assert D2CONFIG_DTYPE.itemsize == 80 and D2DESC_DTYPE.itemsize == 128
+                        + [("filter_id", "<u4"), ("pad", "<i4"), ("drift", "<u8"), ("mpair", "<u8")])
+assert D2CONFIG_DTYPE.itemsize == 80 and D2DESC_DTYPE.itemsize == 136
 
 OUTW = 8
 O_PRED_LL, O_HMEAN, O_FMEAN, O_SMEAN, O_VOL, O_BWF, O_BWS = range(7)
@@ -56,11 +56,14 @@ class TwoFactorBatch:
     drift_budget_gb: the forward keeps every iteration's Stein drift for the
     backward (which then replays positions instead of recomputing the
     transport) when that buffer fits this many GB.
+    record_pairs: keep, per tick and coordinate, the median pair (a, b) the
+    adjoint differentiates through (``self.mpair`` [B, T, 2, 2], -1 where that
+    bandwidth is clamped), filled by the backward.
     """
 
     def __init__(self, y, x0, n_stein=5, checkpoint_every=32, noise=None, seeds=None, lik_offset=0.0,
                  temperature=1.0, rmsprop_rho=0.9, rmsprop_eps=1e-6, paper_sign=False, device="cuda",
-                 drift_budget_gb=32.0):
+                 drift_budget_gb=32.0, record_pairs=False):
         self.L = _lib.load()
         _bind(self.L)
         dev = torch.device(device)
@@ -87,6 +90,7 @@ class TwoFactorBatch:
         # kept when it fits the budget; otherwise the backward recomputes the transport
         drift_gb = B * T * self.S * 2 * n * 4 / 1e9
         self.drift = torch.empty((B, T, self.S, 2, n), **f32) if drift_gb <= drift_budget_gb else None
+        self.mpair = torch.full((B, T, 2, 2), -1, dtype=torch.int32, device=dev) if record_pairs else None
         self.noise = None
         if noise is not None:
             self.noise = {k: torch.as_tensor(noise[k], **f32).contiguous() for k in ("z", "xi")}
@@ -118,6 +122,8 @@ class TwoFactorBatch:
             d["xT"] = ptr(self.xT, b, 2 * n)
             if self.drift is not None:
                 d["drift"] = ptr(self.drift, b, T * self.S * 2 * n)
+            if self.mpair is not None:
+                d["mpair"] = ptr(self.mpair, b, 4 * T)
             d["seed"] = seeds[b]
             d["filter_id"] = b
         self.desc = torch.from_numpy(desc.view(np.uint8).reshape(-1).copy()).to(dev)

@@ -57,10 +57,13 @@ def median_r2(h):
     return torch.kthvalue(r2, k + 1).values
 
 
-def tick(x, v, y_t, y_prev, z, xi, theta, cfg):
+def tick(x, v, y_t, y_prev, z, xi, theta, cfg, pair=None):
     """One tick.  x: positions after the previous tick [N]; v: RMSProp state [N];
     z: predict normals [N]; xi: Langevin normals [S, N]; theta = (mu, rho,
-    sigma_z, nu) as 0-d tensors.  Returns (x_new, v_new, pred_ll)."""
+    sigma_z, nu) as 0-d tensors.  pair: None, or the indices (a, b) of the pair
+    whose squared distance is taken as the median (a < 0: the median itself);
+    a float32 filter and this float64 one can select different pairs of nearly
+    equal distance, whose gradients differ.  Returns (x_new, v_new, pred_ll)."""
     mu, rho, sigma, nu = theta
     n = x.shape[0]
     S = cfg["stein_max_steps"]
@@ -72,7 +75,10 @@ def tick(x, v, y_t, y_prev, z, xi, theta, cfg):
     h = torch.clamp(m + sigma * z, -15.0, 5.0)
     stc = student_t_const(nu)
     pred_ll = logmeanexp(log_w(h, y_t, nu, stc))
-    ell2 = median_r2(h)
+    if pair is not None and pair[0] >= 0:
+        ell2 = (h[pair[0]] - h[pair[1]]) ** 2
+    else:
+        ell2 = median_r2(h)
     bw = torch.clamp(torch.sqrt(ell2), 0.01, 2.0)
     ib = 1.0 / (bw * bw)
     inv_s2 = 1.0 / (sigma * sigma)
@@ -107,16 +113,16 @@ def tick(x, v, y_t, y_prev, z, xi, theta, cfg):
     return h, v, pred_ll
 
 
-def run(theta, y, z, xi, h0, cfg, v0=None, y_prev0=0.0):
-    """T ticks.  y [T]; z [T, N]; xi [T, S, N]; h0 [N].  Returns
-    (pred_ll [T], x_T, v_T).  The loss of the differentiable form is
-    -pred_ll.sum()."""
+def run(theta, y, z, xi, h0, cfg, v0=None, y_prev0=0.0, pairs=None):
+    """T ticks.  y [T]; z [T, N]; xi [T, S, N]; h0 [N]; pairs: None or [T, 2]
+    median pairs (see tick).  Returns (pred_ll [T], x_T, v_T).  The loss of the
+    differentiable form is -pred_ll.sum()."""
     x = h0
     v = torch.zeros_like(h0) if v0 is None else v0
     out = []
     yp = torch.as_tensor(y_prev0, dtype=h0.dtype)
     for t in range(y.shape[0]):
-        x, v, pl = tick(x, v, y[t], yp, z[t], xi[t], theta, cfg)
+        x, v, pl = tick(x, v, y[t], yp, z[t], xi[t], theta, cfg, None if pairs is None else pairs[t])
         out.append(pl)
         yp = y[t]
     return torch.stack(out), x, v

@@ -34,9 +34,19 @@ from .torch_svpf import log_w, logmeanexp, median_r2, student_t_const
 FCLAMP = 20.0
 
 
-def tick(x, v, y_t, z, xi, theta, cfg):
+def _ell2(xc, pair):
+    """The median squared distance of one coordinate, or that of the given pair (a >= 0)."""
+    if pair is not None and pair[0] >= 0:
+        return (xc[pair[0]] - xc[pair[1]]) ** 2
+    return median_r2(xc)
+
+
+def tick(x, v, y_t, z, xi, theta, cfg, pair=None):
     """x [N, 2] positions (f, s); v [N, 2] RMSProp state; z [N, 2]; xi [S, N, 2];
-    theta = (mu, rho_f, sigma_f, rho_s, sigma_s, nu).  Returns (x, v, pred_ll)."""
+    theta = (mu, rho_f, sigma_f, rho_s, sigma_s, nu).  pair: None, or per coordinate
+    the indices (a, b) of the pair taken as that coordinate's median (a < 0: the
+    median itself), to follow the pair choices of a float32 filter.  Returns
+    (x, v, pred_ll)."""
     mu, rho_f, sig_f, rho_s, sig_s, nu = theta
     n = x.shape[0]
     S = cfg["S"]
@@ -47,7 +57,8 @@ def tick(x, v, y_t, z, xi, theta, cfg):
     xp = torch.clamp(m + sig * z, -FCLAMP, FCLAMP)
     stc = student_t_const(nu)
     pred_ll = logmeanexp(log_w(mu + xp[:, 0] + xp[:, 1], y_t, nu, stc))
-    bw = torch.stack([torch.clamp(torch.sqrt(median_r2(xp[:, c])), 0.01, 2.0) for c in range(2)])
+    bw = torch.stack([torch.clamp(torch.sqrt(_ell2(xp[:, c], None if pair is None else pair[c])), 0.01, 2.0)
+                      for c in range(2)])
     ib = 1.0 / (bw * bw)
     Dg = 1.0 / (sig * sig)
     ysq = y_t * y_t
@@ -80,13 +91,14 @@ def tick(x, v, y_t, z, xi, theta, cfg):
     return h, v, pred_ll
 
 
-def run(theta, y, z, xi, x0, cfg, v0=None):
-    """T ticks: y [T], z [T, N, 2], xi [T, S, N, 2], x0 [N, 2] -> (pred_ll [T], x_T, v_T)."""
+def run(theta, y, z, xi, x0, cfg, v0=None, pairs=None):
+    """T ticks: y [T], z [T, N, 2], xi [T, S, N, 2], x0 [N, 2]; pairs: None or [T][2][2]
+    median pairs (see tick) -> (pred_ll [T], x_T, v_T)."""
     x = x0
     v = torch.zeros_like(x0) if v0 is None else v0
     out = []
     for t in range(y.shape[0]):
-        x, v, pl = tick(x, v, y[t], z[t], xi[t], theta, cfg)
+        x, v, pl = tick(x, v, y[t], z[t], xi[t], theta, cfg, None if pairs is None else pairs[t])
         out.append(pl)
     return torch.stack(out), x, v
 

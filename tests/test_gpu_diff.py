@@ -95,6 +95,44 @@ def test_adjoint_matches_torch_autograd(torch_cuda, gamma, sign):
         assert rel_err(g[b], ref) <= 1e-3, (b, g[b], ref)
 
 
+def test_adjoint_at_the_cfg4_shape(torch_cuda):
+    """One filter at N = 1024 over T = 96 ticks with checkpoints every 32 (three full segments):
+    the HIP adjoint against float64 autograd of the dense restatement, run on the GPU (its
+    autograd graph holds about 30 GB).
+
+    The bandwidth is the median of N(N-1)/2 = 523,776 pairwise distances, and its gradient
+    flows through the one selected pair.  Neighbouring order statistics lie about 1e-6 apart,
+    so float32 and float64 arithmetic select different pairs of nearly equal distance, whose
+    gradients differ: torch's own float32 autograd of this restatement is 6.5e-2 off its
+    float64 gradient here, while the float64 gradient moves by 1e-8 when h0 moves by one
+    float32 ulp.  The reference therefore takes the median pairs the float32 forward
+    selected (DiffBatch(record_pairs=True)); everything else is its own float64 arithmetic."""
+    n, T, S, C = 1024, 96, 5, 32
+    th = GRAD_THETAS[0]
+    y, h0, z, xi = inputs(n, T, S, [th], seed=21)
+    bt = DiffBatch(y, h0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi}, record_pairs=True)
+    theta = torch.tensor([th], device="cuda", requires_grad=True)
+    pl32 = predictive_loglik(theta, bt)
+    (-pl32.sum()).backward()
+    g = theta.grad.cpu().numpy()[0]
+    pairs = bt.mpair[0].cpu().numpy()
+    assert (pairs[:, 0] >= 0).all() and (pairs[:, 0] < pairs[:, 1]).all()
+    pl32 = pl32.detach().cpu().numpy()[0]
+    del bt
+    cfg = cfg_of(n, S, th)
+    dt, dev = torch.float64, "cuda"
+    tt = [torch.tensor(v, dtype=dt, device=dev, requires_grad=True) for v in th]
+    pl, _, _ = torch_svpf.run(tt, torch.tensor(y[0], dtype=dt, device=dev), torch.tensor(z[0], dtype=dt, device=dev),
+                              torch.tensor(xi[0], dtype=dt, device=dev), torch.tensor(h0[0], dtype=dt, device=dev),
+                              cfg, pairs=[(int(a), int(b)) for a, b in pairs])
+    (-pl.sum()).backward()
+    ref = np.array([t.grad.item() for t in tt])
+    assert np.max(np.abs(pl32 - pl.detach().cpu().numpy())) <= 1e-4 * max(1.0, float(np.abs(pl32).max()))
+    del pl, tt
+    torch.cuda.empty_cache()
+    assert rel_err(g, ref) <= 1e-3, (g, ref)
+
+
 def test_adjoint_matches_oracle_central_differences(torch_cuda):
     n, T, S = 64, 8, 5
     th = THETAS[0]

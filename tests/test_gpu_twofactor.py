@@ -87,6 +87,62 @@ def test_adjoint_matches_torch_autograd(torch_cuda, sign):
         assert rel_err(g[b], ref) <= 1e-3, (b, g[b], ref)
 
 
+def test_adjoint_matches_central_differences_of_the_restatement(torch_cuda):
+    """The HIP 2-D adjoint against FP64 central differences of the float64 restatement's loss
+    (oracle/torch_svpf2d.py, pinned by its N = 1 Newton step and its own FD check), all six
+    parameters, three checkpoint segments."""
+    n, T, S, C = 64, 8, 5, 3
+    th = GRAD_THETAS[0]
+    bt, (y, x0, z, xi) = make(n, T, S, [th], C, seed=13)
+    theta = torch.tensor([th], device="cuda", requires_grad=True)
+    (-predictive_loglik_2d(theta, bt).sum()).backward()
+    g = theta.grad.cpu().numpy()[0]
+    fd = []
+    for k in range(6):
+        eps = 1e-6
+        tp, tm = list(th), list(th)
+        tp[k] += eps
+        tm[k] -= eps
+        with torch.no_grad():
+            lp = -torch_ref(tp, y[0], x0[0], z[0], xi[0], S)[1].sum().item()
+            lm = -torch_ref(tm, y[0], x0[0], z[0], xi[0], S)[1].sum().item()
+        fd.append((lp - lm) / (2 * eps))
+    fd = np.array(fd)
+    assert rel_err(g, fd) <= 1e-3, (g, fd)
+
+
+def test_adjoint_at_the_cfg5_particle_count(torch_cuda):
+    """One filter at N = 2048 (the two-factor workload's particle count) over 12 ticks with
+    checkpoints every 4: the HIP adjoint against float64 autograd of the restatement, run on
+    the GPU.  Each coordinate's bandwidth is the median of 2.1M pairwise distances whose
+    gradient flows through one selected pair; float32 and float64 select different pairs of
+    nearly equal distance (18% apart in the gradient here when each picks its own), so the
+    reference takes the pairs the float32 forward selected (record_pairs=True)."""
+    n, T, S, C = 2048, 12, 5, 4
+    th = GRAD_THETAS[1]
+    y, x0, z, xi = inputs(n, T, S, [th], seed=17)
+    bt = TwoFactorBatch(y, x0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi}, record_pairs=True)
+    theta = torch.tensor([th], device="cuda", requires_grad=True)
+    (-predictive_loglik_2d(theta, bt).sum()).backward()
+    g = theta.grad.cpu().numpy()[0]
+    pairs = bt.mpair[0].cpu().numpy()
+    assert (pairs[:, :, 0] >= 0).all() and (pairs[:, :, 0] < pairs[:, :, 1]).all()
+    del bt
+    dt, dev = torch.float64, "cuda"
+    tt = [torch.tensor(v, dtype=dt, device=dev, requires_grad=True) for v in th]
+    cfg = dict(torch_svpf2d.DEFAULT_CFG, S=S, paper_sign=0)
+    pl, _, _ = torch_svpf2d.run(tt, torch.tensor(y[0], dtype=dt, device=dev),
+                                torch.tensor(z[0], dtype=dt, device=dev).transpose(1, 2),
+                                torch.tensor(xi[0], dtype=dt, device=dev).transpose(2, 3),
+                                torch.tensor(x0[0], dtype=dt, device=dev).T, cfg,
+                                pairs=[[(int(a), int(b)) for a, b in pc] for pc in pairs])
+    (-pl.sum()).backward()
+    ref = np.array([t.grad.item() for t in tt])
+    del pl, tt
+    torch.cuda.empty_cache()
+    assert rel_err(g, ref) <= 1e-3, (g, ref)
+
+
 def test_gradient_independent_of_checkpoint_interval(torch_cuda):
     """Bit-identical gradients for every checkpoint interval, replaying the
     stored Stein drift or (no drift buffer) recomputing the transport."""
