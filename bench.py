@@ -529,7 +529,10 @@ def time_twofactor(torch, B=1024, N=2048, T=32, C=8):
 
 def calibration_demo(torch, B=128, N=256, T=600, epochs=200, lr=1e-2):
     """Adam on one theta shared by B two-factor filters (each filter runs its own series
-    drawn from the generating theta) from a perturbed start; reports the recovered theta."""
+    drawn from the generating theta) from a perturbed start; reports the recovered theta and
+    the summed loss at the generating, starting and fitted theta: a fitted loss below the
+    generating theta's says the objective's minimum is elsewhere (the objective is at fault),
+    one above it says the optimiser stopped short."""
     from svpf_amd.twofactor import THETA_NAMES, TwoFactorBatch, calibrate
 
     ys, x0 = twofactor_workload(B, N, T, seed=77)
@@ -543,7 +546,8 @@ def calibration_demo(torch, B=128, N=256, T=600, epochs=200, lr=1e-2):
     start[4] *= 1.5
     start[5] = 10.0
     t0 = time.perf_counter()
-    th, hist = calibrate(bt, torch.tensor(start, device="cuda"), epochs=epochs, lr=lr, shared=True)
+    th, hist = calibrate(bt, torch.tensor(start, device="cuda"), epochs=epochs, lr=lr, shared=True,
+                         log=lambda ep, l: log(f"calibration epoch {ep}: mean loss {l:.3f}"))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     est = th.cpu().numpy()
@@ -552,12 +556,15 @@ def calibration_demo(torch, B=128, N=256, T=600, epochs=200, lr=1e-2):
     for name, v in (("true", true), ("start", start), ("estimate", est)):
         pl = bt.forward(torch.tensor(np.tile(v, (B, 1)), device="cuda"))[:, :, 0]
         loss_at[name] = float(-pl.sum())
+    verdict = ("objective: the fitted theta scores better than the generating one"
+               if loss_at["estimate"] < loss_at["true"] else
+               "optimiser: the generating theta scores better than the fitted one")
     return {"workload": f"one theta shared by B={B} filters (own series each), N={N}, T={T}, S={S_ITERS}, "
                         f"Adam lr {lr}, {epochs} epochs",
             "true": dict(zip(THETA_NAMES, true.tolist())), "start": dict(zip(THETA_NAMES, start.tolist())),
             "estimate": dict(zip(THETA_NAMES, est.tolist())),
             "loss_first": float(hist[0].sum()), "loss_last": float(hist[-1].sum()), "loss_at": loss_at,
-            "wall_s": wall}
+            "recovery_limited_by": verdict, "wall_s": wall, "s_per_epoch": wall / epochs}
 
 
 def profile_only(torch, what, B, K, W):
@@ -574,6 +581,8 @@ def profile_only(torch, what, B, K, W):
         log(json.dumps(time_large(torch, 16384, T=K)))
     elif what == "calib":
         print(json.dumps(calibration_demo(torch)), flush=True)
+    elif what == "calib5":
+        print(json.dumps(calibration_demo(torch, B=1024, N=2048, T=1000, epochs=K)), flush=True)
     elif what == "twofactor":
         log(json.dumps(time_twofactor(torch)))
     elif what == "extended":
@@ -598,13 +607,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--B", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
-    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "large", "v2",
+    ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "calib5", "large", "v2",
                                        "extended", "joint"],
                     default=None,
                     help="run one workload once and exit (for profiler runs): the N=2048 tile group of the "
                          "batched workload preceded by a 256 MiB device copy (traffic calibration), the "
                          "differentiable filter, the single N=512 filter, the two-factor filter, the "
-                         "two-factor calibration (prints its JSON), the full-feature N=16384 filter "
+                         "two-factor calibration (prints its JSON; calib5: at B=1024, N=2048, T=1000 for "
+                         "--steps epochs), the full-feature N=16384 filter "
                          "over --steps ticks, the fused single-workgroup step over --steps calls, or the "
                          "Heun / Student-t batched runs over --steps ticks")
     args = ap.parse_args()
@@ -682,6 +692,8 @@ def main():
         extras["fwd_bwd_us_per_step"] = (fwd32 + bwd32) * 1e3 / T_d
         log("two-factor filter B=1024 N=2048")
         extras["twofactor"] = time_twofactor(torch)
+        log("two-factor filter B=1024 N=2048 over T=1000 (forward + adjoint, checkpoints every 32)")
+        extras["twofactor_t1000"] = time_twofactor(torch, T=1000, C=32)
         log("full-feature single filters N=4096 and N=16384, T=2000")
         large = {str(N): time_large(torch, N) for N in (4096, 16384)}
         log("cpu baseline of the full-feature filter (FP64 oracle, N=4096, one thread)")
