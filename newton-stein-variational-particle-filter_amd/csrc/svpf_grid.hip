@@ -17,13 +17,14 @@
 //   grid_median_final 1 x 1024     rounds still needed if any, rank of the candidates -> bandwidth
 //   grid_means       n/256         scaled mixture-component means
 //   per Stein iteration k < max:
-//     grid_gradient  G x 256       mixture prior (component means streamed through LDS tiles) + likelihood
-//                                  + Newton curvature -> transport records {h/bw, s, c/bw, curv}
+//     grid_gradient  G x 256       the KSD and early-stop decision after iteration k - 1 (every block
+//                                  from the same partials); mixture prior (component means streamed
+//                                  through LDS tiles) + likelihood + Newton curvature -> transport
+//                                  records {h/bw, s, c/bw, curv}
 //     grid_transport G x 256       IMQ transport over record tiles in LDS, RMSProp, Langevin noise; KSD
 //                                  partials when the iteration needs them
-//     grid_ksd       1 workgroup   KSD, early-stop decision
 //   grid_outputs     n/256         rejuvenation, output partials
-//   grid_finish      1 workgroup   outputs and the host tail
+//   grid_finish      1 workgroup   the last iteration's KSD decision, outputs and the host tail
 //
 // The Stein budget and early stop are decided on the device: iterations past
 // them return at once, so a tick is a fixed launch sequence (capturable as one
@@ -47,12 +48,24 @@ constexpr int kMaxN = 16384;      // largest filter of this path
 constexpr int kMaxG = 4096;       // most row workgroups of one launch (n / 4 at 64 lanes per row)
 enum { GP_LWMAX = 0, GP_LWSUM = 1, GP_H = 2, GP_H2 = 3, GP_HMIN = 4, GP_HMAX = 5, GP_VOL = 6, GP_KSD = 7, GP_N = 8 };
 
+// The KSD / early-stop state available to Stein iteration k: the KSD of the last
+// iteration that computed one, the previous KSD of the early-stop rule, the stop
+// flag and the iterations run so far.  GridScalars keeps it for k even and odd
+// (dec[k & 1]): the gradient launch of iteration k derives dec[k & 1] from
+// dec[(k - 1) & 1] and iteration k - 1's KSD partials, so no launch reads a slot
+// it writes.
+struct KsdState {
+    float ksd, ksd_last;
+    int stop, total;
+};
+
 struct GridScalars {
     float mu, sigma, y_t, y_prev;
     float gs, guide_mean, guide_std, ib;
     float bw, bw_sq, rs, sq;
-    float inv_s2, pred_ll, ksd, ksd_last;
-    int budget, stop, total, pad;
+    float inv_s2, pred_ll;
+    int budget, pad;
+    KsdState dec[2];
 };
 
 // beta and step of global Stein iteration it under the budget (the engine's loop order)
@@ -282,10 +295,44 @@ __global__ void grid_prologue(const svpf_fparams* __restrict__ Pp, svpf_fstate* 
     // 1 / sqrt(nu_s sigma^2) for the Student-t one
     g->sq = (F & SVPF_F_STUDENT_STATE) ? sqrtf(g->inv_s2 / P.nu_state) : sqrtf(1.4426950408889634f * 0.5f * g->inv_s2);
     g->budget = budget;
-    g->stop = 0;
-    g->total = 0;
-    g->ksd = 0.0f;
-    g->ksd_last = -1.0f;
+    g->dec[0] = KsdState{0.0f, -1.0f, 0, 0};
+    g->dec[1] = g->dec[0];
+}
+
+// The state available to Stein iteration k (see KsdState): for k > 0 the decision
+// after iteration k - 1 from dec[(k - 1) & 1] and that iteration's KSD partials
+// kpart[(k - 1) & 1][0, G) (src/svpf_optimized_graph.cu:1261-1290).  Every block of
+// a launch computes it the same way (a fixed-order sum), so all take the same
+// decision.  Called by every thread of a kGB-thread block.
+SVPF_DEV KsdState ksd_state(const svpf_fparams& P, const svpf_fstate& S, const GridScalars* g,
+                            const float* __restrict__ kpart, int k, int G, int n, float* red) {
+    if (k == 0) return g->dec[0];
+    const int j = k - 1;
+    const KsdState prev = g->dec[j & 1];
+    KsdState st = prev;
+    if (j >= g->budget || prev.stop) return st;  // iteration j did not run
+    float beta;
+    bool last;
+    iter_schedule(P, S, g->budget, j, beta, last);
+    const bool early = (P.flags & SVPF_F_KSD_EARLY_STOP) != 0;
+    st.total = j + 1;
+    if (!(last || early)) return st;
+    float s[1] = {0.f};
+    const float* kp = kpart + (size_t)(j & 1) * kMaxG;
+    for (int b = threadIdx.x; b < G; b += kGB) s[0] += kp[b];
+    block_sum<kGB, 1>(s, red);
+    const float inv_n = 1.0f / (float)n;
+    const float ksd = sqrtf(fmaxf(s[0] * inv_n * inv_n, 0.0f));
+    st.ksd = ksd;
+    if (early && prev.ksd_last > 0.0f && j + 1 >= P.stein_min) {
+        const float rel = (prev.ksd_last - ksd) / prev.ksd_last;
+        if (rel < P.ksd_thr) {
+            st.stop = 1;
+            return st;
+        }
+    }
+    st.ksd_last = ksd;
+    return st;
 }
 
 // workgroup reduction of GP_N partials written by thread 0 to part[blockIdx.x]
@@ -726,12 +773,17 @@ __global__ __launch_bounds__(kGB) void grid_means(const svpf_fparams* __restrict
 template <int CS, bool ST>
 __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restrict__ Pp,
                                                      const svpf_fstate* __restrict__ Sp,
-                                                     const GridScalars* __restrict__ g, const float* __restrict__ h,
+                                                     GridScalars* __restrict__ g, const float* __restrict__ h,
                                                      const float* __restrict__ mm, float4* __restrict__ rec,
-                                                     float* __restrict__ lw, float* __restrict__ save, int it, int n,
+                                                     float* __restrict__ lw, float* __restrict__ save,
+                                                     const float* __restrict__ kpart, int it, int G, int n,
                                                      int npad) {
     __shared__ __align__(16) float tile[kMeanTile];
-    if (it >= g->budget || g->stop) return;  // uniform across the grid
+    __shared__ float kred[64];
+    // the early-stop decision after the previous iteration (uniform across the grid)
+    const KsdState st = ksd_state(*Pp, *Sp, g, kpart, it, G, n, kred);
+    if (it > 0 && blockIdx.x == 0 && threadIdx.x == 0) g->dec[it & 1] = st;
+    if (it >= g->budget || st.stop) return;
     const svpf_fparams& P = *Pp;
     const uint32_t F = P.flags;
     const int i = blockIdx.x * (kGB / CS) + threadIdx.x / CS, cs = threadIdx.x % CS;
@@ -921,7 +973,8 @@ __global__ __launch_bounds__(kGB) void grid_transport(const svpf_fparams* __rest
                                                       float* __restrict__ v, float* __restrict__ part, int it, int t,
                                                       int n) {
     __shared__ __align__(16) float4 tile[kRecTile];
-    if (it >= g->budget || g->stop) return;
+    if (it >= g->budget || g->dec[it & 1].stop) return;
+    part += (size_t)(it & 1) * kMaxG;  // this iteration's KSD partials
     const svpf_fparams& P = *Pp;
     const uint32_t F = P.flags;
     float beta;
@@ -955,7 +1008,8 @@ __global__ __launch_bounds__(kGB) void grid_heun_transport(const svpf_fparams* _
                                                            const float* __restrict__ ho, float* __restrict__ phib,
                                                            int it, int t, int n) {
     __shared__ __align__(16) float4 tile[kRecTile];
-    if (it >= g->budget || g->stop) return;
+    if (it >= g->budget || g->dec[it & 1].stop) return;
+    part += (size_t)(it & 1) * kMaxG;  // this iteration's KSD partials
     const svpf_fparams& P = *Pp;
     const uint32_t F = P.flags;
     float beta;
@@ -977,7 +1031,7 @@ __global__ __launch_bounds__(kGB) void grid_heun_ksd_records(const svpf_fparams*
                                                              const GridScalars* __restrict__ g,
                                                              const float* __restrict__ ho, float4* __restrict__ rec,
                                                              int it, int n) {
-    if (it >= g->budget || g->stop) return;
+    if (it >= g->budget || g->dec[it & 1].stop) return;
     float beta;
     bool last;
     iter_schedule(*Pp, *Sp, g->budget, it, beta, last);
@@ -995,7 +1049,7 @@ __global__ __launch_bounds__(kGB) void grid_heun_correct(const svpf_fparams* __r
                                                          const float* __restrict__ ho, const float* __restrict__ phib,
                                                          float* __restrict__ h, float* __restrict__ v, int it, int t,
                                                          int n) {
-    if (it >= g->budget || g->stop) return;
+    if (it >= g->budget || g->dec[it & 1].stop) return;
     const int i = blockIdx.x * kGB + threadIdx.x;
     if (i >= n) return;
     const svpf_fparams& P = *Pp;
@@ -1010,36 +1064,6 @@ __global__ __launch_bounds__(kGB) void grid_heun_correct(const svpf_fparams* __r
     if (temp > 1e-6f) diff = sqrtf(2.0f * eff * temp) * grid_xi(*Dp, *Sp, i, it, t, n);
     v[i] = vn;
     h[i] = clamp_logvol(ho[i] + eff * p * rsqrtf(vn + P.rms_eps) + diff);
-}
-
-__global__ __launch_bounds__(kGB) void grid_ksd(const svpf_fparams* __restrict__ Pp,
-                                                const svpf_fstate* __restrict__ Sp, GridScalars* __restrict__ g,
-                                                const float* __restrict__ part, int it, int G, int n) {
-    __shared__ float red[64];
-    if (it >= g->budget || g->stop) return;
-    const svpf_fparams& P = *Pp;
-    float beta;
-    bool last;
-    iter_schedule(P, *Sp, g->budget, it, beta, last);
-    const bool early = (P.flags & SVPF_F_KSD_EARLY_STOP) != 0;
-    float s[1] = {0.f};
-    if (last || early)
-        for (int b = threadIdx.x; b < G; b += kGB) s[0] += part[b];
-    block_sum<kGB, 1>(s, red);
-    if (threadIdx.x != 0) return;
-    g->total = it + 1;
-    if (!(last || early)) return;
-    const float inv_n = 1.0f / (float)n;
-    const float ksd = sqrtf(fmaxf(s[0] * inv_n * inv_n, 0.0f));
-    g->ksd = ksd;
-    if (early && g->ksd_last > 0.0f && it + 1 >= P.stein_min) {
-        const float rel = (g->ksd_last - ksd) / g->ksd_last;
-        if (rel < P.ksd_thr) {
-            g->stop = 1;
-            return;
-        }
-    }
-    g->ksd_last = ksd;
 }
 
 __global__ __launch_bounds__(kGB) void grid_outputs(const svpf_fparams* __restrict__ Pp,
@@ -1080,10 +1104,12 @@ __global__ __launch_bounds__(kGB) void grid_outputs(const svpf_fparams* __restri
 __global__ __launch_bounds__(kGB) void grid_finish(const svpf_fparams* __restrict__ Pp, svpf_fstate* __restrict__ Sp,
                                                    const svpf_fdesc* __restrict__ Dp, int t,
                                                    const GridScalars* __restrict__ g, const float* __restrict__ part,
-                                                   int nb, int n) {
+                                                   const float* __restrict__ kpart, int stein_max, int G, int nb,
+                                                   int n) {
     __shared__ float red[64];
     const svpf_fparams& P = *Pp;
     svpf_fstate& S = *Sp;
+    const KsdState fin = ksd_state(P, S, g, kpart, stein_max, G, n, red);
     float M = -3.0e38f;
     for (int b = threadIdx.x; b < nb; b += kGB) M = fmaxf(M, part[(size_t)b * GP_N + GP_LWMAX]);
     M = block_max<kGB>(M, red);
@@ -1099,8 +1125,8 @@ __global__ __launch_bounds__(kGB) void grid_finish(const svpf_fparams* __restric
     const float inv_n = 1.0f / (float)n;
     const float ll = M + logf(fmaxf(sums[0] / (float)n, 1e-10f));
     const float vol = sums[1] * inv_n, hm = sums[2] * inv_n;
-    const float ksd = g->ksd;
-    const int total = g->total;
+    const float ksd = fin.ksd;
+    const int total = fin.total;
     const svpf_fdesc& D = *Dp;
     if (D.out) {
         float* o = D.out + (size_t)t * SVPF_OUTW;
@@ -1146,7 +1172,7 @@ int grid_cs(int n) {
 // Workspace of one filter (offsets in floats, each a multiple of 4):
 //   rec [n] float4 | mm [npad] | part [(nb + kMaxG) x GP_N] | scalars [32] | srt [n + 4] | runs [n] |
 //   plo [n] int | phi [n] int | mstate [2] MedState + gcount (32 floats) | mpart [2][kMaxMG][kMW] uint32 |
-//   mcand [kMedCand]
+//   mcand [kMedCand] | ho [n] | phib [n] | kpart [2][kMaxG]
 struct GridWs {
     float4* rec;
     float* mm;
@@ -1160,8 +1186,9 @@ struct GridWs {
     uint32_t* gcount;
     uint32_t* mpart;
     float* mcand;
-    float* ho;    // Heun: positions at the iteration start
-    float* phib;  // Heun: phi1, then (phi1 + phi2) / 2
+    float* ho;     // Heun: positions at the iteration start
+    float* phib;   // Heun: phi1, then (phi1 + phi2) / 2
+    float* kpart;  // [2][kMaxG] KSD block partials of the even / odd Stein iterations
 };
 
 size_t grid_ws_layout(int n, float* base, GridWs* w) {
@@ -1194,6 +1221,8 @@ size_t grid_ws_layout(int n, float* base, GridWs* w) {
     o += r4(n);
     const size_t o_phib = o;
     o += r4(n);
+    const size_t o_kpart = o;
+    o += 2 * (size_t)kMaxG;
     static_assert(sizeof(GridScalars) <= 32 * sizeof(float) && 2 * sizeof(MedState) <= 28 * sizeof(float), "ws slots");
     if (w) {
         w->rec = reinterpret_cast<float4*>(base + o_rec);
@@ -1210,6 +1239,7 @@ size_t grid_ws_layout(int n, float* base, GridWs* w) {
         w->mcand = base + o_mc;
         w->ho = base + o_ho;
         w->phib = base + o_phib;
+        w->kpart = base + o_kpart;
     }
     return o;
 }
@@ -1248,16 +1278,22 @@ template <int CS, bool ST>
 void grid_iteration(const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, const GridWs& W, float* h, float* v,
                     float* lw, bool heun, int G, int nb, int it, int t, int n, int npad, hipStream_t stream) {
     if (!heun) {
-        grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, nullptr, it, n, npad);
-        grid_transport<CS><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, it, t, n);
+        grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, nullptr, W.kpart, it, G, n,
+                                                     npad);
+        grid_transport<CS><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.kpart, it, t, n);
         return;
     }
-    grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, W.ho, it, n, npad);
-    grid_heun_transport<CS, PH_PRED><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, W.ho, W.phib, it, t, n);
-    grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, nullptr, it, n, npad);
-    grid_heun_transport<CS, PH_PHI2><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, W.ho, W.phib, it, t, n);
+    // both gradient launches of a Heun iteration take the same decision (the partials of iteration
+    // it - 1 are untouched until the KSD pass of iteration it)
+    grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, W.ho, W.kpart, it, G, n, npad);
+    grid_heun_transport<CS, PH_PRED><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.kpart, W.ho, W.phib, it, t,
+                                                            n);
+    grid_gradient<CS, ST><<<G, kGB, 0, stream>>>(P, S, W.gsc, h, W.mm, W.rec, lw, nullptr, W.kpart, it, G, n, npad);
+    grid_heun_transport<CS, PH_PHI2><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.kpart, W.ho, W.phib, it, t,
+                                                            n);
     grid_heun_ksd_records<<<nb, kGB, 0, stream>>>(P, S, W.gsc, W.ho, W.rec, it, n);
-    grid_heun_transport<CS, PH_KSD><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.part, W.ho, W.phib, it, t, n);
+    grid_heun_transport<CS, PH_KSD><<<G, kGB, 0, stream>>>(P, S, D, W.gsc, W.rec, h, v, W.kpart, W.ho, W.phib, it, t,
+                                                           n);
     grid_heun_correct<<<nb, kGB, 0, stream>>>(P, S, D, W.gsc, W.ho, W.phib, h, v, it, t, n);
 }
 
@@ -1267,8 +1303,8 @@ void grid_iteration(const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, 
         else grid_iteration<CSV, false>(P, S, D, W, h, v, lw, heun, G, nb, it, t, n, npad, stream);              \
         break;
 
-// Ticks [t0, t1) of one filter on `stream`: per tick 10 + med_rounds(n) + 3 stein_max launches
-// (9 per iteration with the Heun step; stein_max bounds the device-side Stein budget).
+// Ticks [t0, t1) of one filter on `stream`: per tick 10 + med_rounds(n) + 2 stein_max launches
+// (7 per iteration with the Heun step; stein_max bounds the device-side Stein budget).
 // h, hp, v, lw: the filter's particle arrays [n]; ws: svpf_grid_workspace_floats(n) floats;
 // ext: SVPF_GRID_EXT_HEUN / SVPF_GRID_EXT_STUDENT as the filter's flags have them.
 extern "C" hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, const svpf_fdesc* D, float* h,
@@ -1310,10 +1346,9 @@ extern "C" hipError_t svpf_launch_grid(const svpf_fparams* P, svpf_fstate* S, co
                 SVPF_GRID_ITER(64)
                 default: return hipErrorInvalidValue;
             }
-            grid_ksd<<<1, kGB, 0, stream>>>(P, S, W.gsc, W.part, it, G, n);
         }
         grid_outputs<<<nb, kGB, 0, stream>>>(P, S, D, t, W.gsc, h, lw, W.part, n);
-        grid_finish<<<1, kGB, 0, stream>>>(P, S, D, t, W.gsc, W.part, nb, n);
+        grid_finish<<<1, kGB, 0, stream>>>(P, S, D, t, W.gsc, W.part, W.kpart, stein_max, G, nb, n);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

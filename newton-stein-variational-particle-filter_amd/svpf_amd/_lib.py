@@ -165,6 +165,14 @@ def load(path: str | None = None):
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise RuntimeError(f"libsvpf_amd.so not built ({p}); run __graft_entry__.build()")
+    # PyTorch ships its own HIP runtime under the same soname (libamdhip64.so.7) as the
+    # ROCm one this library links.  Whichever loads first serves the process; loading this
+    # library first would leave torch with a second runtime that finds no GPU.  Load torch's
+    # first, so torch tensors and this library share one runtime whatever the import order.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(p)
     SP = ctypes.POINTER(SVPFState)
     PP = ctypes.POINTER(SVPFParams)
