@@ -13,16 +13,29 @@
 // Phase timestamps for latency analysis (off unless built with -DSVPF_TRACE):
 // thread 0 of block 0 stores the 100 MHz wall clock into slot k of the
 // tick's 64-slot row.
+// SVPF_TRACE_ROUND(k) stamps slot 51 + k of the tick last stamped at slot 0 (the
+// rounds of the median selection).
 #ifdef SVPF_TRACE
 extern __device__ unsigned long long svpf_trace_buf[256 * 64];
+extern __device__ unsigned svpf_trace_tick;
 #define SVPF_TRACE_POINT(tick, k)                                                         \
     do {                                                                                  \
-        if (threadIdx.x == 0 && blockIdx.x == 0)                                          \
+        if (threadIdx.x == 0 && blockIdx.x == 0) {                                        \
             svpf_trace_buf[(((unsigned)(tick)) & 255u) * 64u + ((k) & 63)] = wall_clock64(); \
+            if ((k) == 0) svpf_trace_tick = (unsigned)(tick);                             \
+        }                                                                                 \
+    } while (0)
+#define SVPF_TRACE_ROUND(k)                                                                              \
+    do {                                                                                                 \
+        if (threadIdx.x == 0 && blockIdx.x == 0 && (k) < 12)                                             \
+            svpf_trace_buf[(svpf_trace_tick & 255u) * 64u + 51u + (unsigned)(k)] = wall_clock64();       \
     } while (0)
 #else
 #define SVPF_TRACE_POINT(tick, k) \
     do {                          \
+    } while (0)
+#define SVPF_TRACE_ROUND(k) \
+    do {                    \
     } while (0)
 #endif
 
@@ -599,7 +612,9 @@ SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi
             cH = chi;
         }
         ++round;
+        SVPF_TRACE_ROUND(round);
     }
+    SVPF_TRACE_ROUND(11);
     if (L >= H) return __uint_as_float(H);
     // gather the cH - cL candidates (pairs with r^2 in [L, H]) and select by rank
     uint32_t cnt = 0u;

@@ -147,6 +147,7 @@ extern "C" int svpf_coop_rows(int cs) { return 256 / cs; }
 
 #ifdef SVPF_TRACE
 __device__ unsigned long long svpf_trace_buf[256 * 64];
+__device__ unsigned svpf_trace_tick;
 extern "C" hipError_t svpf_trace_dump(unsigned long long* out) {
     return hipMemcpyFromSymbol(out, HIP_SYMBOL(svpf_trace_buf), sizeof(unsigned long long) * 256 * 64);
 }
