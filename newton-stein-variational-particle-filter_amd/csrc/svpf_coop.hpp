@@ -60,7 +60,7 @@ struct CoopEngine {
     float* hfin;     // [n]  final positions of the previous tick (this tick's h_prev)
     float* hpp;      // [n]  final positions two ticks back (MIM asymmetric rho)
     float* lwall;    // [n]  predictive log weights (all particles)
-    float* red;      // [64]
+    float* red;      // [kRedWords]
     TickScalars* sc;
     int* flag;
 
@@ -69,7 +69,7 @@ struct CoopEngine {
 
     __host__ __device__ static size_t lds_bytes(int nmax) {
         int p2 = LdsLayout::mbuf_len(nmax);
-        size_t b = (size_t)nmax * 16 + (size_t)p2 * 4 + 4 * (size_t)nmax * 4 + 64 * 4 + sizeof(TickScalars) + 16;
+        size_t b = (size_t)nmax * 16 + (size_t)p2 * 4 + 4 * (size_t)nmax * 4 + kRedWords * 4 + sizeof(TickScalars) + 16;
         return (b + 15) & ~(size_t)15;
     }
 
@@ -85,7 +85,7 @@ struct CoopEngine {
         hpp = hfin + n;
         lwall = hpp + n;
         red = lwall + n;
-        sc = reinterpret_cast<TickScalars*>(red + 64);
+        sc = reinterpret_cast<TickScalars*>(red + kRedWords);
         flag = reinterpret_cast<int*>(sc + 1);
         rg = threadIdx.x / CS;
         cs = threadIdx.x % CS;
@@ -308,7 +308,8 @@ struct CoopEngine {
                     // gradient of the own row (src/svpf_opt_kernels.cu:354-521)
                     float sr[1], srd[1];
                     E.h[0] = h;
-                    E.template prior_sums<false>(sq, sr, srd);
+                    const float own_mm[1] = {valid() ? mbuf[row()] : h * sq};  // the row's own component
+                    E.template prior_sums<false, true>(sq, sr, srd, own_mm);
                     float sj, cj, cv;
                     row_gradient(P, beta, y_t, inv_s2, sq, sgn2, mode, sr[0], srd[0], sj, cj, cv);
                     SVPF_TRACE_POINT(t, 8 + 3 * it_row);

@@ -46,6 +46,9 @@ constexpr int kMedianCands = 32;  // candidate pairs gathered for the final rank
 // thresholds counted per selection round (15 measured slower inside one workgroup:
 // the wider lockstep searches cost more per round than the rounds they save)
 constexpr int kMedianW = 7;
+// LDS words of the reduction scratch `red`: the median selection uses 48 + BS/64 + 1
+// (65 at 1024 threads), the block reductions fewer.
+constexpr int kRedWords = 80;
 
 // ---------------------------------------------------------------------------
 // Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as easy

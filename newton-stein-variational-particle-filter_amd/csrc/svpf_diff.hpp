@@ -76,7 +76,7 @@ struct DiffLds {
     // the record area also holds the final 4 x 16 doubles of the theta-adjoint reduction
     __host__ __device__ static int rec_len(int nmax) { return nmax < 32 ? 32 : nmax; }
     __host__ __device__ static size_t bytes(int nmax) {
-        size_t b = (size_t)(rec_len(nmax) + nmax) * 16 + (size_t)mbuf_len(nmax) * 4 + 64 * 4 + 16 * 4;
+        size_t b = (size_t)(rec_len(nmax) + nmax) * 16 + (size_t)mbuf_len(nmax) * 4 + kRedWords * 4 + 16 * 4;
         return (b + 15) & ~(size_t)15;
     }
 };
@@ -87,7 +87,7 @@ struct DiffEngine {
     float4* recb;   // [nmax] pass-B records {scaled h, log2 normaliser, a, kappa E dd}; predicted h in the pair
                     // search; the sorted means during the forward Stein loop
     float* mbuf;    // [p2] scaled mixture means (padded with far sentinels) / sorted positions
-    float* red;     // [64] reduction scratch
+    float* red;     // [kRedWords] reduction scratch
     float* scal;    // [16] broadcast scalars
     int n, p2;
     float inv_n;
@@ -100,7 +100,7 @@ struct DiffEngine {
         recb = rec + DiffLds::rec_len(nmax);
         mbuf = reinterpret_cast<float*>(recb + nmax);
         red = mbuf + DiffLds::mbuf_len(nmax);
-        scal = red + 64;
+        scal = red + kRedWords;
         inv_n = 1.0f / (float)n;
     }
     SVPF_DEV static int row(int r) { return (int)threadIdx.x + r * BS; }

@@ -58,7 +58,7 @@ SVPF_DEV D2Theta d2_theta(const float* th) {
 struct D2Lds {
     __host__ __device__ static int rec_len(int nmax) { return nmax < 32 ? 32 : nmax; }
     __host__ __device__ static size_t bytes(int nmax) {
-        size_t b = (size_t)rec_len(nmax) * 16 * 3 + (size_t)DiffLds::mbuf_len(nmax) * 8 + 64 * 4 + 16 * 4;
+        size_t b = (size_t)rec_len(nmax) * 16 * 3 + (size_t)DiffLds::mbuf_len(nmax) * 8 + kRedWords * 4 + 16 * 4;
         return (b + 15) & ~(size_t)15;
     }
 };
@@ -84,7 +84,7 @@ struct D2Engine {
         rec3 = rec2 + rl;
         mbuf = reinterpret_cast<float*>(rec3 + rl);
         red = mbuf + 2 * DiffLds::mbuf_len(nmax);
-        scal = red + 64;
+        scal = red + kRedWords;
         inv_n = 1.0f / (float)n;
     }
     SVPF_DEV static int row(int r) { return (int)threadIdx.x + r * BS; }

@@ -53,7 +53,7 @@ struct LdsLayout {
         int p2 = mbuf_len(nmax);
         size_t b = (size_t)nmax * 16;          // hsc records
         b += (size_t)p2 * 4;                   // mixture means / sort buffer
-        b += 64 * 4;                           // reduction scratch
+        b += kRedWords * 4;                    // reduction scratch
         b += sizeof(TickScalars);
         return (b + 15) & ~(size_t)15;
     }
@@ -77,7 +77,7 @@ struct Engine {
 
     float4* hsc;       // [n]      {h, s, c, curv}
     float* mbuf;       // [p2]     mixture means (and sort buffer during the bandwidth phase)
-    float* red;        // [64]
+    float* red;        // [kRedWords]
     TickScalars* sc;
 
     int n, p2, rg, cs;
@@ -91,7 +91,7 @@ struct Engine {
         hsc = reinterpret_cast<float4*>(lds);
         mbuf = reinterpret_cast<float*>(lds + (size_t)nmax * 16);
         red = mbuf + p2max;
-        sc = reinterpret_cast<TickScalars*>(red + 64);
+        sc = reinterpret_cast<TickScalars*>(red + kRedWords);
         rg = threadIdx.x / CS;
         cs = threadIdx.x % CS;
     }
@@ -183,14 +183,28 @@ struct Engine {
 
     // SORTED: mbuf holds the means sorted (sort_means) and the shift is a binary
     // search; otherwise (unsorted means) a first pass over the means finds it:
-    // sub + mul + min per pair.
-    template <bool SORTED = true>
-    SVPF_DEV void prior_sums(float sq, float (&sr)[R], float (&srd)[R]) const {
+    // sub + mul + min per pair.  OWN (unsorted means): own_mm[r] is the scaled mean
+    // of the component row r was predicted from.  Any shift at or above the nearest
+    // one gives the same sums up to rounding while no term overflows, and the own
+    // component's squared distance is one: its term is 2^0 and every term at most
+    // 2^own.  The first pass then runs only in waves with a row whose own component
+    // lies more than 10 scaled units away (own > 100).
+    template <bool SORTED = true, bool OWN = false>
+    SVPF_DEV void prior_sums(float sq, float (&sr)[R], float (&srd)[R], const float* own_mm = nullptr) const {
         float hh[R], off[R];
+        bool far = false;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             hh[r] = h[r] * sq;
-            off[r] = SORTED ? nearest_sq(hh[r]) : 3.0e38f;
+            if (SORTED) {
+                off[r] = nearest_sq(hh[r]);
+            } else if (OWN) {
+                const float d = hh[r] - own_mm[r];
+                off[r] = d * d;
+                far = far || off[r] > 100.0f;
+            } else {
+                off[r] = 3.0e38f;
+            }
             sr[r] = 0.f;
             srd[r] = 0.f;
         }
@@ -198,7 +212,11 @@ struct Engine {
         const float4* m4 = reinterpret_cast<const float4*>(mbuf);
         // U loads in flight per step: at one wave per SIMD nothing else hides the LDS latency
         constexpr int U = R >= 2 ? 2 : 4;
-        if constexpr (!SORTED) {
+        if (!SORTED && (!OWN || __any(far))) {
+            if (OWN) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) off[r] = 3.0e38f;
+            }
             auto pass1 = [&](const float4& m) {
 #pragma unroll
                 for (int r = 0; r < R; ++r) {

@@ -794,13 +794,19 @@ __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restr
     const float hi = i < n ? h[i] : 0.f;
     const float hh = hi * sq;
     const float a_t = 0.5f * (P.nu_state + 1.0f);
-    // pass 1: nearest component (the reference's max shift); pass 2: sums
-    float off = 3.0e38f, sr = 0.f, srd = 0.f, shp = 0.f;
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) {
-            off = group_min<CS>(off);
-            if (ST) off = a_t * __builtin_amdgcn_logf(1.0f + off);
-        }
+    // The shift of the log-responsibilities: the reference takes the nearest component's
+    // (a first pass over all means).  Any shift at or above it gives the same sums up to
+    // rounding as long as no term overflows, and the particle's own component (it was
+    // predicted from it) is one: its term is 2^0 and every term is at most 2^own.  So the
+    // first pass runs only in blocks with a row whose own component lies more than 10
+    // scaled units away (own > 100).
+    const float d_own = i < n ? hh - mm[i] : 0.0f;
+    const float own = ST ? a_t * __builtin_amdgcn_logf(fmaf(d_own, d_own, 1.0f)) : d_own * d_own;
+    const bool exact = __syncthreads_or(own > 100.0f) != 0;
+    // pass 1 (blocks with a far row only): nearest component, the reference's max shift;
+    // pass 2: the sums.  Each pass streams the means through LDS in tiles.
+    float off = exact ? 3.0e38f : own, sr = 0.f, srd = 0.f, shp = 0.f;
+    auto sweep = [&](auto&& body) {
         for (int j0 = 0; j0 < npad; j0 += kMeanTile) {
             const int len = min(kMeanTile, npad - j0);
             __syncthreads();
@@ -808,33 +814,48 @@ __global__ __launch_bounds__(kGB) void grid_gradient(const svpf_fparams* __restr
                 *reinterpret_cast<float4*>(tile + k) = *reinterpret_cast<const float4*>(mm + j0 + k);
             __syncthreads();
             const float4* t4 = reinterpret_cast<const float4*>(tile);
-            for (int q = cs; q < (len >> 2); q += CS) {
-                const float4 m = t4[q];
-                const float d0 = hh - m.x, d1 = hh - m.y, d2 = hh - m.z, d3 = hh - m.w;
-                if (pass == 0) {
-                    off = fminf(off, fminf(fminf(d0 * d0, d1 * d1), fminf(d2 * d2, d3 * d3)));
-                } else if (ST) {
-                    const float dd[4] = {d0, d1, d2, d3};
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const float q = fmaf(dd[c], dd[c], 1.0f);
-                        const float iq = __builtin_amdgcn_rcpf(q);
-                        const float e = __builtin_amdgcn_exp2f(fmaf(-a_t, __builtin_amdgcn_logf(q), off));
-                        const float u = e * iq;
-                        sr += e;
-                        srd = fmaf(u, dd[c], srd);
-                        shp = fmaf(u * iq, 2.0f - q, shp);
-                    }
-                } else {
-                    const float e0 = __builtin_amdgcn_exp2f(fmaf(-d0, d0, off));
-                    const float e1 = __builtin_amdgcn_exp2f(fmaf(-d1, d1, off));
-                    const float e2 = __builtin_amdgcn_exp2f(fmaf(-d2, d2, off));
-                    const float e3 = __builtin_amdgcn_exp2f(fmaf(-d3, d3, off));
-                    sr += (e0 + e1) + (e2 + e3);
-                    srd = fmaf(e0, d0, fmaf(e1, d1, fmaf(e2, d2, fmaf(e3, d3, srd))));
-                }
+            const int nq = len >> 2;
+            int q = cs;
+            for (; q + CS < nq; q += 2 * CS) {  // two loads in flight
+                const float4 m0 = t4[q], m1 = t4[q + CS];
+                body(m0);
+                body(m1);
             }
+            if (q < nq) body(t4[q]);
         }
+    };
+    if (exact) {
+        sweep([&](const float4& m) {
+            const float d0 = hh - m.x, d1 = hh - m.y, d2 = hh - m.z, d3 = hh - m.w;
+            off = fminf(off, fminf(fminf(d0 * d0, d1 * d1), fminf(d2 * d2, d3 * d3)));
+        });
+        off = group_min<CS>(off);
+        if (ST) off = a_t * __builtin_amdgcn_logf(1.0f + off);
+    }
+    if (ST) {
+        sweep([&](const float4& m) {
+            const float dd[4] = {hh - m.x, hh - m.y, hh - m.z, hh - m.w};
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float q = fmaf(dd[c], dd[c], 1.0f);
+                const float iq = __builtin_amdgcn_rcpf(q);
+                const float e = __builtin_amdgcn_exp2f(fmaf(-a_t, __builtin_amdgcn_logf(q), off));
+                const float u = e * iq;
+                sr += e;
+                srd = fmaf(u, dd[c], srd);
+                shp = fmaf(u * iq, 2.0f - q, shp);
+            }
+        });
+    } else {
+        sweep([&](const float4& m) {
+            const float d0 = hh - m.x, d1 = hh - m.y, d2 = hh - m.z, d3 = hh - m.w;
+            const float e0 = __builtin_amdgcn_exp2f(fmaf(-d0, d0, off));
+            const float e1 = __builtin_amdgcn_exp2f(fmaf(-d1, d1, off));
+            const float e2 = __builtin_amdgcn_exp2f(fmaf(-d2, d2, off));
+            const float e3 = __builtin_amdgcn_exp2f(fmaf(-d3, d3, off));
+            sr += (e0 + e1) + (e2 + e3);
+            srd = fmaf(e0, d0, fmaf(e1, d1, fmaf(e2, d2, fmaf(e3, d3, srd))));
+        });
     }
     sr = group_sum<CS>(sr);
     srd = group_sum<CS>(srd);
@@ -900,25 +921,41 @@ SVPF_DEV void grid_transport_rows(const svpf_fparams& P, const svpf_fstate& S, c
         __syncthreads();
         for (int k = threadIdx.x; k < len; k += kGB) tile[k] = rec[j0 + k];
         __syncthreads();
-        for (int j = cs; j < len; j += CS) {
-            const float4 e = tile[j];
+        // In scaled positions the IMQ satisfies d^2 K = 1 - K, so the Stein kernel's factor
+        // 4 d^2 K - 1 is 3 - 4K and the full-Newton weight K^2 |3 d^2 - 1| is K |3 - 4K|;
+        // on a KSD pass the transport reuses the KSD sums: acc = a1 + sum_j c_j rs d K^2.
+        auto body = [&](const float4& e) {
             const float d = ht - e.x;
             const float K = __builtin_amdgcn_rcpf(fmaf(d, d, 1.0f));
-            acc = fmaf(K, fmaf(e.z * d, K, e.y), acc);
-            if (MODE == MODE_FULL) {
-                const float K2 = K * K;
-                hw = fmaf(e.w, K, hw);
-                hw = fmaf(two_ib * K2, fabsf(fmaf(3.0f * d, d, -1.0f)), hw);
-                kn += K;
-            }
             if (KSD) {
                 const float K2 = K * K, u = d * K2;
                 a1 = fmaf(K, e.y, a1);
                 a2 += u;
                 a3 = fmaf(e.y, u, a3);
-                a4 = fmaf(K2, fmaf(4.0f * d * d, K, -1.0f), a4);
+                acc = fmaf(e.z, u, acc);  // repulsion; the attraction a1 joins after the loop
+                const float tk = fmaf(-4.0f, K, 3.0f);
+                a4 = fmaf(K2, tk, a4);
+                if (MODE == MODE_FULL) {
+                    hw = fmaf(e.w, K, hw);
+                    hw = fmaf(two_ib, K * fabsf(tk), hw);
+                    kn += K;
+                }
+            } else {
+                acc = fmaf(K, fmaf(e.z * d, K, e.y), acc);
+                if (MODE == MODE_FULL) {
+                    hw = fmaf(e.w, K, hw);
+                    hw = fmaf(two_ib, K * fabsf(fmaf(-4.0f, K, 3.0f)), hw);
+                    kn += K;
+                }
             }
+        };
+        int j = cs;
+        for (; j + CS < len; j += 2 * CS) {  // two loads in flight
+            const float4 e0 = tile[j], e1 = tile[j + CS];
+            body(e0);
+            body(e1);
         }
+        if (j < len) body(tile[j]);
     }
     acc = group_sum<CS>(acc);
     if (MODE == MODE_FULL) {
@@ -930,6 +967,7 @@ SVPF_DEV void grid_transport_rows(const svpf_fparams& P, const svpf_fstate& S, c
         a2 = group_sum<CS>(a2);
         a3 = group_sum<CS>(a3);
         a4 = group_sum<CS>(a4);
+        acc += a1;
     }
     float ksdp = 0.f;
     if (i < n && cs == 0) {
