@@ -316,6 +316,13 @@ int svpf_get_coop_shape(const SVPFState* state, int* lanes_per_row, int* workgro
  * testing, at least -2^20). */
 int svpf_coop_launch_is_cooperative(const SVPFState* state);
 int svpf_set_coop_spin_limit(SVPFState* state, int polls);
+/* No reference counterpart: with the median bandwidth (bandwidth_mode
+ * SVPF_BANDWIDTH_MEDIAN) the cooperative kernel runs one more workgroup that
+ * sorts the predicted particles and selects the median pairwise distance while
+ * the row workgroups compute the first Stein iteration's gradient and exchange
+ * its records; on = 0 makes every row workgroup select the median itself
+ * instead.  Results are identical either way.  Default on. */
+int svpf_set_coop_median_workgroup(SVPFState* state, int on);
 /* Failure-path testing: the given workgroup of the cooperative kernel (-1: none)
  * gives up the waits of a launch's final exchange after 4 polls, after publishing
  * its rows, while the others complete; the launch then commits nothing. */

@@ -90,6 +90,9 @@ struct svpf_impl {
     bool captured_cooperative = false;
     int coop_spin_limit = 0;
     int coop_fault_wg = -1;  // failure-path testing (svpf_set_coop_fault)
+    // median bandwidth: a workgroup beside the row workgroups selects the median while they run the
+    // first gradient (svpf_set_coop_median_workgroup)
+    bool coop_bwg = true;
     // grid path (workspace allocated for 2 <= n <= SVPF_GRID_MAX_PARTICLES)
     int grid_mode = SVPF_GRID_AUTO;
     float* dgrid_ws = nullptr;
@@ -160,7 +163,9 @@ bool choose_coop_shape(svpf_impl* I, int cs, int stride) {
     const int rows = 256 / cs;
     const int G = (n + rows - 1) / rows;
     // partial sums of all workgroups (G x 8 words) are gathered into an n-float LDS array
-    const bool fits = svpf_coop_lds_bytes(n) <= 159 * 1024 && G * 8 <= n && G <= n / 8 + 2 && G * stride <= 2048;
+    // (G + 1: the median workgroup)
+    const bool fits =
+        svpf_coop_lds_bytes(n) <= 159 * 1024 && G * 8 <= n && G <= n / 8 + 2 && (G + 1) * stride <= 2048;
     if (!fits) return false;
     I->coop_cs = cs;
     I->coop_G = G;
@@ -412,6 +417,14 @@ bool use_grid(const SVPFState* s) {
 
 bool use_coop(const SVPFState* s) { return !use_grid(s) && coop_applies(s); }
 
+// The median workgroup runs beside the row workgroups when it fits beside them on one
+// XCD at one workgroup per CU (an XCD has 32 CUs): the plain launch relies on every
+// workgroup being resident at once.
+bool coop_median_wg(const SVPFState* s) {
+    const svpf_impl* I = s->impl;
+    return I->coop_bwg && s->bandwidth_mode == SVPF_BW_MEDIAN && I->coop_G + 1 <= 32;
+}
+
 // launch-sequence variants of the grid path
 unsigned grid_ext(const SVPFState* s) {
     return (s->use_heun ? SVPF_GRID_EXT_HEUN : 0u) | (s->use_student_t_state ? SVPF_GRID_EXT_STUDENT : 0u);
@@ -430,7 +443,8 @@ hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
                                 I->n, s->stein_max_steps, grid_ext(s), t0, t1, cs);
     if (use_coop(s))
         return svpf_launch_coop(I->coop_cs, I->coop_G, I->coop_stride, I->dp(), I->ds(), I->dd(), I->d_h(),
-                                I->d_hp(), I->d_v(), I->xbuf, I->n, t0, t1, I->coop_cooperative ? 1 : 0, cs);
+                                I->d_hp(), I->d_v(), I->xbuf, I->n, t0, t1, I->coop_cooperative ? 1 : 0,
+                                coop_median_wg(s) ? 1 : 0, cs);
     const int tile = I->tile | ((s->use_heun || s->use_student_t_state) ? SVPF_TILE_EXT : 0);
     return svpf_launch_batched(I->bs, tile, 1, I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->dorder,
                                1, nullptr, I->n, t0, t1, cs);
@@ -1219,6 +1233,14 @@ int svpf_set_coop_shape(SVPFState* s, int lanes_per_row, int xcd_stride) {
     I->coop_G = trial.coop_G;
     I->coop_stride = trial.coop_stride;
     I->coop_ok = true;
+    svpf_graph_invalidate(s);
+    return set_err(s, SVPF_RC_OK);
+}
+
+int svpf_set_coop_median_workgroup(SVPFState* s, int on) {
+    if (!s || !s->impl) return SVPF_RC_INVALID_ARGUMENT;
+    if (on != 0 && on != 1) return set_err(s, SVPF_RC_INVALID_ARGUMENT);
+    s->impl->coop_bwg = on != 0;
     svpf_graph_invalidate(s);
     return set_err(s, SVPF_RC_OK);
 }

@@ -32,17 +32,20 @@ struct CoopIO {
     float* v;
     uint64_t* xbuf;   // [2][4][n] transport records, [2][n] final positions, [2][G][PS_N] partials, then the
                       // commit area (svpf_coop_xbuf_words)
-    int G;            // cooperating workgroups
-    int g;            // this workgroup
+    int G;            // cooperating workgroups of the particle rows
+    int g;            // this workgroup (g == G: the median workgroup, bwg launches only)
+    int bwg;          // a median workgroup runs beside the G row workgroups
 };
 
 // Exchange buffer of one filter of n particles, in 64-bit words: the records,
 // final positions and partials (G <= n / 8 + 2), then the commit stage -- each
-// row's RMSProp state and log-weight as tagged words, 2n -- and one commit counter.
+// row's RMSProp state and log-weight as tagged words, 2n -- one commit counter,
+// and [2] words of the median workgroup's pairwise-distance median.
 __host__ __device__ inline size_t svpf_coop_stage_offset(int n) {
     return 10 * (size_t)n + 2 * (size_t)PS_N * ((size_t)n / 8 + 2);
 }
-__host__ __device__ inline size_t svpf_coop_xbuf_words(int n) { return svpf_coop_stage_offset(n) + 2 * (size_t)n + 1; }
+__host__ __device__ inline size_t svpf_coop_med_offset(int n) { return svpf_coop_stage_offset(n) + 2 * (size_t)n + 1; }
+__host__ __device__ inline size_t svpf_coop_xbuf_words(int n) { return svpf_coop_med_offset(n) + 2; }
 
 SVPF_DEV uint64_t tag_word(float v, uint32_t epoch) { return ((uint64_t)epoch << 32) | (uint64_t)__float_as_uint(v); }
 
@@ -98,17 +101,25 @@ struct CoopEngine {
     // Spin until words [0, cnt) of buf show `epoch`; store values through put(k, value).
     template <class Put>
     SVPF_DEV bool gather(const uint64_t* buf, int cnt, uint32_t epoch, Put put) {
+        return gather_at(cnt, epoch, [buf](int k) { return buf + k; }, put);
+    }
+    // Words [0, cnt) at addresses addr(k).  Eight loads per thread are in flight at
+    // once, so an exchange of up to 8 BS words (the records of n = 512 with full
+    // Newton) costs one L2 round trip.
+    template <class Addr, class Put>
+    SVPF_DEV bool gather_at(int cnt, uint32_t epoch, Addr addr, Put put) {
+        constexpr int U = 8;
         bool ok = true;
-        for (int k0 = threadIdx.x; k0 < cnt; k0 += 4 * BS) {
-            uint64_t w[4];
+        for (int k0 = threadIdx.x; k0 < cnt; k0 += U * BS) {
+            uint64_t w[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 int k = k0 + u * BS;
-                w[u] = k < cnt ? __hip_atomic_load(buf + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                w[u] = k < cnt ? __hip_atomic_load(addr(k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                                : ((uint64_t)(epoch ^ epoch_xor) << 32);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 int k = k0 + u * BS;
                 long spins = 0;
                 while ((uint32_t)(w[u] >> 32) != (epoch ^ epoch_xor)) {
@@ -117,7 +128,7 @@ struct CoopEngine {
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
-                    w[u] = __hip_atomic_load(buf + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    w[u] = __hip_atomic_load(addr(k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 if (!ok) break;
                 if (k < cnt) put(k, __uint_as_float((uint32_t)w[u]));
@@ -129,6 +140,20 @@ struct CoopEngine {
         bool all_ok = (*flag == 0);
         __syncthreads();
         return all_ok;
+    }
+
+    // The final exchange: n positions from xf into hfin, then G x PS_N partials from xp
+    // into lwall (free by then).
+    SVPF_DEV bool gather_final(const uint64_t* xf, const uint64_t* xp, uint32_t epoch) {
+        float* hf = hfin;
+        float* pr = lwall;
+        const int nn = n;
+        return gather_at(
+            n + G * PS_N, epoch, [xf, xp, nn](int k) { return k < nn ? xf + k : xp + (k - nn); },
+            [hf, pr, nn](int k, float val) {
+                if (k < nn) hf[k] = val;
+                else pr[k - nn] = val;
+            });
     }
 
     SVPF_DEV void publish(uint64_t* p, float v, uint32_t epoch) {
@@ -181,6 +206,12 @@ struct CoopEngine {
         uint64_t* xrec = io.xbuf;                          // [2][4][n]
         uint64_t* xfin = io.xbuf + 8 * (size_t)n;           // [2][n]
         uint64_t* xpart = xfin + 2 * (size_t)n;             // [2][G][PS_N]
+        uint64_t* xmed = io.xbuf + svpf_coop_med_offset(n);  // [2]
+        // bwg: the median workgroup (g == G) sorts the predicted particles and selects the
+        // median pairwise distance while the row workgroups run the first gradient and
+        // exchange its records unscaled; they scale them once the median arrives
+        const bool bwg = io.bwg != 0 && P.bandwidth_mode == SVPF_BW_MEDIAN;
+        const bool medwg = bwg && g == G;
         const float inv_n = 1.0f / (float)n;
         const int mode = (F & SVPF_F_NEWTON) ? ((F & SVPF_F_FULL_NEWTON) ? MODE_FULL : MODE_NEWTON) : MODE_PLAIN;
         const int nrec = (mode == MODE_FULL) ? 4 : 3;
@@ -261,13 +292,33 @@ struct CoopEngine {
             }
             __syncthreads();
             SVPF_TRACE_POINT(t, 1);
+            const uint32_t ep_med = epoch;  // the tick's median word carries the first records epoch
+            if (medwg) {
+                // ---- the median workgroup: the bandwidth, then the row workgroups' final exchange ----
+                SVPF_TRACE_STAMP(t, 44);
+                bandwidth(P, fs, y_t, t);
+                if (threadIdx.x == 0) publish(xmed + (ep_med & 1u), fs.med_r2, ep_med);
+                SVPF_TRACE_STAMP(t, 45);
+                // its scalar state follows the row workgroups' exactly, so it can commit the launch
+                const float pred_ll_m = lds_logmeanexp(lwall);
+                epoch += (uint32_t)sc->budget;  // one records exchange per Stein iteration of the rows
+                const uint32_t ep = epoch++;
+                for (int i = threadIdx.x; i < n; i += BS) hpp[i] = hfin[i];
+                __syncthreads();
+                if (!gather_final(xfin + (size_t)(ep & 1u) * n, xpart + (size_t)(ep & 1u) * G * PS_N, ep))
+                    return false;
+                if (threadIdx.x < kWave) finish_from_partials(P, fs, D, t, pred_ll_m, sc->budget);
+                __syncthreads();
+                continue;
+            }
             const float pred_ll = lds_logmeanexp(lwall);
             h = valid() ? hall[row()] : 0.f;
             SVPF_TRACE_POINT(t, 2);
 
-            // ---- bandwidth (every workgroup, all particles) ----
-            bandwidth(P, fs, y_t, t);
-            const float ib = sc->ib, rs = sc->rs;
+            // ---- bandwidth (every workgroup, all particles; with bwg the median workgroup's) ----
+            bool bw_ready = !bwg;
+            if (!bwg) bandwidth(P, fs, y_t, t);
+            float ib = sc->ib, rs = sc->rs;  // with bwg: set when the median arrives
             SVPF_TRACE_POINT(t, 3);
 
             // ---- scaled mixture means (unsorted: at this size a sort per tick costs more than
@@ -317,9 +368,9 @@ struct CoopEngine {
                     uint64_t* xr = xrec + (size_t)(ep & 1u) * 4 * n;
                     if (cs == 0 && valid()) {
                         int i = row();
-                        publish(xr + i, h * rs, ep);
+                        publish(xr + i, bw_ready ? h * rs : h, ep);
                         publish(xr + n + i, sj, ep);
-                        publish(xr + 2 * (size_t)n + i, cj * rs, ep);
+                        publish(xr + 2 * (size_t)n + i, bw_ready ? cj * rs : cj, ep);
                         if (nrec == 4) publish(xr + 3 * (size_t)n + i, cv, ep);
                     }
                     float4* rec = hsc;
@@ -329,6 +380,19 @@ struct CoopEngine {
                             reinterpret_cast<float*>(&rec[i])[a] = val;
                         }))
                         return false;
+                    if (!bw_ready) {
+                        // the records went out unscaled: scale them once the median is in
+                        // (one rounding each, as when scaled before publishing)
+                        if (!await_bandwidth(xmed, ep_med, fs)) return false;
+                        ib = sc->ib;
+                        rs = sc->rs;
+                        for (int i = threadIdx.x; i < n; i += BS) {
+                            hsc[i].x *= rs;
+                            hsc[i].z *= rs;
+                        }
+                        __syncthreads();
+                        bw_ready = true;
+                    }
                     SVPF_TRACE_POINT(t, 9 + 3 * it_row);
                     // transport of the own row
                     float phi, kp;
@@ -358,6 +422,8 @@ struct CoopEngine {
                     SVPF_TRACE_POINT(t, 10 + 3 * it_row);
                 }
             }
+
+            if (!bw_ready && !await_bandwidth(xmed, ep_med, fs)) return false;  // no Stein iteration ran
 
             // ---- rejuvenation (src/svpf_opt_kernels.cu:1248-1275) ----
             if ((F & SVPF_F_REJUV) && (F & SVPF_F_GUIDE) && fs.timestep > 10 && fs.ksd_prev > P.rej_thr && valid()) {
@@ -405,26 +471,10 @@ struct CoopEngine {
                 epoch_xor = 0x80000000u;
                 spin_limit = 4;
             }
-            float* hf = hfin;
-            if (!gather(xf, n, ep, [hf](int k, float val) { hf[k] = val; })) return false;
+            // final positions and partials in one gather (one round of polls)
+            if (!gather_final(xf, xp, ep)) return false;
             SVPF_TRACE_POINT(t, 41);
-            float* pr = lwall;  // reused: G x PS_N partials
-            if (!gather(xp, G * PS_N, ep, [pr](int k, float val) { pr[k] = val; })) return false;
-            if (threadIdx.x == 0) {
-                float M = -3.0e38f, vs = 0.f, hs = 0.f, ks = 0.f, ls = 0.f;
-                for (int q = 0; q < G; ++q) M = fmaxf(M, pr[q * PS_N + PS_LWMAX]);
-                for (int q = 0; q < G; ++q) {
-                    vs += pr[q * PS_N + PS_VOL];
-                    hs += pr[q * PS_N + PS_H];
-                    ks += pr[q * PS_N + PS_KSD];
-                    float mq = pr[q * PS_N + PS_LWMAX];
-                    if (mq > -1.0e38f) ls += pr[q * PS_N + PS_LWSUM] * expf(mq - M);
-                }
-                float ll = M + logf(fmaxf(ls * inv_n, 1e-10f));
-                float vol = vs * inv_n, hm = hs * inv_n;
-                float ksd = sqrtf(fmaxf(ks * inv_n * inv_n, 0.0f));
-                finish(P, fs, D, t, ll, vol, hm, ksd, pred_ll, total);
-            }
+            if (threadIdx.x < kWave) finish_from_partials(P, fs, D, t, pred_ll, total);
             __syncthreads();
             SVPF_TRACE_POINT(t, 42);
         }
@@ -450,7 +500,9 @@ struct CoopEngine {
             // workgroup arriving here has gathered: one atomic add each, no retries
             const uint64_t before = __hip_atomic_fetch_add(commit, (uint64_t)1, __ATOMIC_RELAXED,
                                                            __HIP_MEMORY_SCOPE_AGENT);
-            last_wg = (before == (uint64_t)(G - 1)) ? 1 : 0;
+            // the median workgroup arrives too (it has no rows to stage): a launch whose median
+            // workgroup timed out commits nothing either
+            last_wg = (before == (uint64_t)(G + (bwg ? 1 : 0) - 1)) ? 1 : 0;
         }
         __syncthreads();
         if (last_wg) {
@@ -468,6 +520,53 @@ struct CoopEngine {
             }
         }
         return true;
+    }
+
+    // The median workgroup's pairwise-distance median of this tick (epoch ep): the
+    // bandwidth and its state, as bandwidth() sets them.  All threads call it.
+    SVPF_DEV bool await_bandwidth(const uint64_t* xmed, uint32_t ep, svpf_fstate& S) {
+        TickScalars* t = sc;
+        if (!gather(xmed + (ep & 1u), 1, ep, [t](int, float val) { t->bw_sq = val; })) return false;
+        if (threadIdx.x == 0) {
+            const float med = sc->bw_sq;
+            S.med_r2 = med;
+            const float bw = fminf(fmaxf(sqrtf(med), 0.01f), 2.0f);
+            sc->bw = bw;
+            sc->ib = 1.0f / (bw * bw);
+            sc->rs = 1.0f / bw;
+        }
+        __syncthreads();
+        return true;
+    }
+
+    // Outputs of the tick from the G row workgroups' partials (in lwall) and the host
+    // tail.  Called by the 64 lanes of wave 0: lane l reduces workgroups l, l + 64, ...
+    SVPF_DEV void finish_from_partials(const svpf_fparams& P, svpf_fstate& S, const svpf_fdesc& D, int t,
+                                       float pred_ll, int total) {
+        const float* pr = lwall;
+        const float inv_n = 1.0f / (float)n;
+        const int l = threadIdx.x;
+        float M = -3.0e38f;
+        for (int q = l; q < G; q += kWave) M = fmaxf(M, pr[q * PS_N + PS_LWMAX]);
+        M = wave_max(M);
+        float vs = 0.f, hs = 0.f, ks = 0.f, ls = 0.f;
+        for (int q = l; q < G; q += kWave) {
+            vs += pr[q * PS_N + PS_VOL];
+            hs += pr[q * PS_N + PS_H];
+            ks += pr[q * PS_N + PS_KSD];
+            float mq = pr[q * PS_N + PS_LWMAX];
+            if (mq > -1.0e38f) ls += pr[q * PS_N + PS_LWSUM] * expf(mq - M);
+        }
+        vs = wave_sum(vs);
+        hs = wave_sum(hs);
+        ks = wave_sum(ks);
+        ls = wave_sum(ls);
+        if (l == 0) {
+            float ll = M + logf(fmaxf(ls * inv_n, 1e-10f));
+            float vol = vs * inv_n, hm = hs * inv_n;
+            float ksd = sqrtf(fmaxf(ks * inv_n * inv_n, 0.0f));
+            finish(P, S, D, t, ll, vol, hm, ksd, pred_ll, total);
+        }
     }
 
     // Row gradient (src/svpf_opt_kernels.cu:354-521); sets lw and s of the own row.
@@ -618,7 +717,7 @@ struct CoopEngine {
             SVPF_TRACE_POINT(t, 50);
             int* plo = reinterpret_cast<int*>(hsc);
             float med = median_pairwise_r2<BS, 2>(mbuf, n, S.med_r2, plo, plo + n, reinterpret_cast<float*>(plo + 2 * n),
-                                                  min(kMedianCands, 2 * n), reinterpret_cast<uint32_t*>(red));
+                                                  2 * n, reinterpret_cast<uint32_t*>(red));
             if (threadIdx.x == 0) {
                 S.med_r2 = med;
                 float bw = fminf(fmaxf(sqrtf(med), 0.01f), 2.0f);

@@ -30,6 +30,11 @@ extern __device__ unsigned svpf_trace_tick;
         if (threadIdx.x == 0 && blockIdx.x == 0 && (k) < 12)                                             \
             svpf_trace_buf[(svpf_trace_tick & 255u) * 64u + 51u + (unsigned)(k)] = wall_clock64();       \
     } while (0)
+// thread 0 of the calling workgroup stamps slot k of the tick (for a workgroup other than 0)
+#define SVPF_TRACE_STAMP(tick, k)                                                                \
+    do {                                                                                         \
+        if (threadIdx.x == 0) svpf_trace_buf[(((unsigned)(tick)) & 255u) * 64u + ((k) & 63)] = wall_clock64(); \
+    } while (0)
 #else
 #define SVPF_TRACE_POINT(tick, k) \
     do {                          \
@@ -37,12 +42,17 @@ extern __device__ unsigned svpf_trace_tick;
 #define SVPF_TRACE_ROUND(k) \
     do {                    \
     } while (0)
+#define SVPF_TRACE_STAMP(tick, k) \
+    do {                          \
+    } while (0)
 #endif
 
 namespace svpf {
 
 constexpr int kWave = 64;
-constexpr int kMedianCands = 32;  // candidate pairs gathered for the final rank selection
+constexpr int kMedianCands = 32;  // candidate pairs of the final rank selection by counting
+constexpr int kHistBins = 256;    // bins of a histogram pass over the gathered candidates
+constexpr int kHistCands = 1024;  // candidates gathered when the scratch holds the bins too
 // thresholds counted per selection round (15 measured slower inside one workgroup:
 // the wider lockstep searches cost more per round than the rounds they save)
 constexpr int kMedianW = 7;
@@ -484,13 +494,18 @@ SVPF_DEV void row_search(const float* s, int n, float si, int a, int b, const ui
 // squared interquartile range, which the median of r^2 equals for normal
 // data); while the rank stays below (above) all of them the next round
 // extends the grid downwards (upwards) with steps 8x wider, and once it is
-// bracketed the bracket is split evenly.  Once at
-// most `cmax` candidate pairs remain they are gathered into `cand` and the
-// rank is read off by counting.  Scratch (LDS): plo, phi (n ints each), cand
-// (cmax floats), red (48 + BS/64 + 1 words); Idx may be a 16-bit type when n < 65536.  The first KR row slots of each thread keep
-// their search results in registers; later slots search again.
+// bracketed the bracket is split evenly.  Once few enough candidate pairs
+// remain they are gathered: with cap >= kHistBins + 64 words of scratch at scr,
+// up to kHistCands, which histogram passes (kHistBins equal bins of bit
+// patterns over the bracket, LDS atomics, one prefix scan) narrow to at most
+// kMedianCands -- a pass over a thousand gathered values costs less than a
+// round of binary searches over every row; with less scratch, kMedianCands.
+// The rank is then read off by counting.  Scratch (LDS): plo, phi (n Idx
+// each), scr (cap words), red (48 + BS/64 + 1 words); Idx may be a 16-bit type
+// when n < 65536.  The first KR row slots of each thread keep their search
+// results in registers; later slots search again.
 template <int BS, int KR, typename Idx = int, int W = kMedianW>
-SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi, float* cand, int cmax,
+SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi, float* scr, int cap,
                                   uint32_t* red) {
     static_assert(W <= 16, "counts: three rotating 16-word buffers");
     constexpr int NW = BS / kWave;
@@ -503,6 +518,12 @@ SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi
     const int rows = n - 1;
     const uint32_t M = (uint32_t)(((uint64_t)n * (uint64_t)(n - 1)) / 2);
     const uint32_t need = (M - 1u) / 2u + 1u;  // count(<= x) must reach rank + 1
+    // scratch: with room for the bins, up to kHistCands candidates are gathered and
+    // narrowed by histogram passes; otherwise kMedianCands, ranked by counting
+    const bool hsel = cap >= kHistBins + 2 * kMedianCands;
+    uint32_t* hist = reinterpret_cast<uint32_t*>(scr);
+    float* cand = hsel ? scr + kHistBins : scr;
+    const int cmax = hsel ? min(cap - kHistBins, kHistCands) : min(cap, kMedianCands);
     const float dmax = s[n - 1] - s[0];
     uint32_t L = 0u, H = __float_as_uint(dmax * dmax);
     uint32_t cL = 0u, cH = M;  // pairs below L, pairs at or below H
@@ -636,7 +657,65 @@ SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi
         }
     }
     __syncthreads();
-    const int C = (int)(cH - cL);
+    int C = (int)(cH - cL);
+    if (hsel && C > kMedianCands) {
+        // histogram passes over the candidates: kHistBins equal bins of bit patterns over
+        // [L, H]; the bin holding the rank becomes [L, H] until at most kMedianCands remain
+        const int C0 = C;
+        bool first = true;
+        while (cH - cL > (uint32_t)kMedianCands && L < H) {
+            const uint32_t span = H - L;  // bins cover [L, L + (kHistBins << sh)) >= [L, H]
+            int sh = 0;
+            while (((uint64_t)kHistBins << sh) <= (uint64_t)span) ++sh;
+            if (!first) __syncthreads();  // every wave has read the previous pass's bins
+            first = false;
+            for (int k = threadIdx.x; k < kHistBins; k += BS) hist[k] = 0u;
+            __syncthreads();
+            for (int k = threadIdx.x; k < C0; k += BS) {
+                const uint32_t x = __float_as_uint(cand[k]);
+                if (x >= L && x <= H) atomicAdd(&hist[(x - L) >> sh], 1u);
+            }
+            __syncthreads();
+            // every wave scans the bins (identical results): lane l holds bins 4l .. 4l+3
+            static_assert(kHistBins == 4 * kWave, "four bins per lane");
+            const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                           h3 = hist[4 * lane + 3];
+            const uint32_t incl = wave_scan_u32(h0 + h1 + h2 + h3);
+            const uint64_t hit = __ballot(cL + incl >= need);
+            // the bins hold every candidate of [L, H], so some lane reaches the rank
+            const int ls = __builtin_amdgcn_readfirstlane(hit ? (int)__builtin_ctzll(hit) : kWave - 1);
+            uint32_t base = cL + (uint32_t)__builtin_amdgcn_readlane((int)(incl - (h0 + h1 + h2 + h3)), ls);
+            const uint32_t g[4] = {(uint32_t)__builtin_amdgcn_readlane((int)h0, ls),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)h1, ls),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)h2, ls),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)h3, ls)};
+            int b = 0;
+            while (b < 3 && base + g[b] < need) base += g[b++];
+            const uint32_t nL = L + ((uint32_t)(4 * ls + b) << sh);
+            const uint32_t top = nL + ((1u << sh) - 1u);
+            H = top < H ? top : H;
+            L = nL;
+            cL = base;
+            cH = base + g[b];
+        }
+        if (L >= H) return __uint_as_float(H);
+        // compact the candidates left in [L, H] into the bins' words
+        __syncthreads();
+        if (threadIdx.x == 0) red[0] = 0u;
+        __syncthreads();
+        float* small = reinterpret_cast<float*>(hist);
+        for (int k = threadIdx.x; k < C0; k += BS) {
+            const float v = cand[k];
+            const uint32_t x = __float_as_uint(v);
+            if (x >= L && x <= H) {
+                const uint32_t q = atomicAdd(&red[0], 1u);
+                if (q < (uint32_t)kHistBins) small[q] = v;
+            }
+        }
+        __syncthreads();
+        cand = small;
+        C = (int)(cH - cL);
+    }
     const uint32_t r0 = need - cL - 1u;
     for (int k = threadIdx.x; k < C; k += BS) {
         const float v = cand[k];

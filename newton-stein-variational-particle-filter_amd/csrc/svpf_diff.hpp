@@ -242,7 +242,7 @@ struct DiffEngine {
         int* plo = reinterpret_cast<int*>(rec);
         constexpr int KR = R < 2 ? 2 : R;
         float med = median_pairwise_r2<BS, KR>(mbuf, n, est, plo, plo + n, reinterpret_cast<float*>(plo + 2 * n),
-                                               min(kMedianCands, 2 * n), reinterpret_cast<uint32_t*>(red));
+                                               2 * n, reinterpret_cast<uint32_t*>(red));
         __syncthreads();
         return med;
     }

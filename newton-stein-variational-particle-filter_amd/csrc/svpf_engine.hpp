@@ -873,7 +873,7 @@ struct Engine {
             block_sort<BS>(tmp, mbuf, n);
             int* plo = reinterpret_cast<int*>(hsc);
             float med = median_pairwise_r2<BS, 2>(mbuf, n, sc->med_est, plo, plo + n, reinterpret_cast<float*>(plo + 2 * n),
-                                                  min(kMedianCands, 2 * n), reinterpret_cast<uint32_t*>(red));
+                                                  2 * n, reinterpret_cast<uint32_t*>(red));
             if (threadIdx.x == 0) {
                 S.med_r2 = med;
                 float bw = fminf(fmaxf(sqrtf(med), 0.01f), 2.0f);

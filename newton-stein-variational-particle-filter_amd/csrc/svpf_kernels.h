@@ -50,7 +50,7 @@ size_t svpf_coop_lds_bytes(int n);
 size_t svpf_coop_xbuf_words_host(int n);
 hipError_t svpf_launch_coop(int cs, int G, int stride, const svpf_fparams* params, svpf_fstate* fstate,
                             const svpf_fdesc* desc, float* pool_h, float* pool_hp, float* pool_v, uint64_t* xbuf,
-                            int n, int t0, int t1, int cooperative, hipStream_t stream);
+                            int n, int t0, int t1, int cooperative, int bwg, hipStream_t stream);
 #ifdef __cplusplus
 }
 #endif

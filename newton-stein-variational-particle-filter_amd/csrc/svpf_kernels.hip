@@ -122,14 +122,14 @@ __global__ __launch_bounds__(BS) void svpf_coop_kernel(const svpf_fparams* __res
                                                        svpf_fstate* __restrict__ fstate,
                                                        const svpf_fdesc* __restrict__ desc, float* pool_h,
                                                        float* pool_hp, float* pool_v, uint64_t* xbuf, int G,
-                                                       int stride, int t0, int t1) {
+                                                       int stride, int bwg, int t0, int t1) {
     extern __shared__ __align__(16) char lds[];
     if ((int)blockIdx.x % stride) return;
     const int g = (int)blockIdx.x / stride;
-    if (g >= G) return;
+    if (g > G || (g == G && !bwg)) return;  // g == G: the median workgroup
     const svpf_fdesc* D = desc;
     const svpf_fparams* P = &params[D->param_idx];
-    CoopIO io{P, fstate, D, pool_h + D->off, pool_hp + D->off, pool_v + D->off, xbuf, G, g};
+    CoopIO io{P, fstate, D, pool_h + D->off, pool_hp + D->off, pool_v + D->off, xbuf, G, g, bwg};
     CoopEngine<BS, CS> e;
     e.bind(lds, P->n, G, g);
     // a timed-out wait: the status word is set atomically and never written by the commit,
@@ -158,7 +158,8 @@ extern "C" size_t svpf_coop_xbuf_words_host(int n) { return svpf_coop_xbuf_words
 
 extern "C" hipError_t svpf_launch_coop(int cs, int G, int stride, const svpf_fparams* params, svpf_fstate* fstate,
                                        const svpf_fdesc* desc, float* pool_h, float* pool_hp, float* pool_v,
-                                       uint64_t* xbuf, int n, int t0, int t1, int cooperative, hipStream_t stream) {
+                                       uint64_t* xbuf, int n, int t0, int t1, int cooperative, int bwg,
+                                       hipStream_t stream) {
     const void* fn = nullptr;
     size_t lds = 0;
     switch (cs) {
@@ -170,13 +171,14 @@ extern "C" hipError_t svpf_launch_coop(int cs, int G, int stride, const svpf_fpa
         SVPF_COOP_CASE(32)
         default: return hipErrorInvalidValue;
     }
-    if (G < 1 || stride < 1 || (long)G * (256 / cs) < n || G * 8 > n) return hipErrorInvalidValue;
+    if (G < 1 || stride < 1 || (long)G * (256 / cs) < n || G * 8 > n || (bwg != 0 && bwg != 1))
+        return hipErrorInvalidValue;
     if (lds > 159 * 1024) return hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
     void* args[] = {(void*)&params, (void*)&fstate, (void*)&desc, (void*)&pool_h, (void*)&pool_hp, (void*)&pool_v,
-                    (void*)&xbuf, (void*)&G, (void*)&stride, (void*)&t0, (void*)&t1};
-    dim3 grid(G * stride), block(256);
+                    (void*)&xbuf, (void*)&G, (void*)&stride, (void*)&bwg, (void*)&t0, (void*)&t1};
+    dim3 grid((G + bwg) * stride), block(256);
     if (cooperative) return hipLaunchCooperativeKernel(fn, grid, block, args, (unsigned)lds, stream);
     return hipLaunchKernel(fn, grid, block, args, lds, stream);
 }

@@ -107,6 +107,7 @@ EXPORTED = (
     "svpf_set_grid_path", "svpf_get_path", "svpf_fully_fused_step_v2", "svpf_fully_fused_v2_supported",
     "svpf_coop_launch_is_cooperative", "svpf_set_coop_spin_limit", "svpf_digamma_f32",
     "svpf_get_two_factor_particles", "svpf_run_sequence_device_noalloc", "svpf_set_coop_fault",
+    "svpf_set_coop_median_workgroup",
     # score diagnostics (include/svpf.h)
     "svpf_gradient_diagnostic_create", "svpf_gradient_diagnostic_destroy", "svpf_compute_nu_diagnostic",
     "svpf_compute_nu_diagnostic_simple", "svpf_compute_sigma_diagnostic_simple", "svpf_snapshot_particles",
@@ -245,6 +246,8 @@ def load(path: str | None = None):
     L.svpf_set_coop_fault.argtypes = [SP, c_int]
     L.svpf_set_coop_spin_limit.restype = c_int
     L.svpf_set_coop_spin_limit.argtypes = [SP, c_int]
+    L.svpf_set_coop_median_workgroup.restype = c_int
+    L.svpf_set_coop_median_workgroup.argtypes = [SP, c_int]
     L.svpf_get_coop_shape.argtypes = [SP, ctypes.POINTER(c_int), ctypes.POINTER(c_int), ctypes.POINTER(c_int)]
     kinds = {"p": c_void_p, "f": c_float, "i": c_int, "noise": ctypes.POINTER(SVPFV2Noise)}
     L.svpf_fully_fused_step_v2.restype = c_int

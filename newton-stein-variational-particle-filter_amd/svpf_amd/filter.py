@@ -23,12 +23,15 @@ def _f32(a):
 class Filter:
     """A single SVPF filter driven through libsvpf_amd.so."""
 
-    def __init__(self, cfg: dict, stream=None, path: str = "auto", coop_shape=None):
+    def __init__(self, cfg: dict, stream=None, path: str = "auto", coop_shape=None, median_workgroup=True):
         """path: "auto" (the library's choice by n and features: one workgroup
         below n = 256, the cooperative multi-workgroup kernel for 256 <= n <
         1024 where it applies, else the grid path), "single" (one workgroup),
         "coop" (cooperative kernel when the features allow it, else one
-        workgroup) or "grid" (per-tick sequence of grid-wide kernels, n >= 2)."""
+        workgroup) or "grid" (per-tick sequence of grid-wide kernels, n >= 2).
+        median_workgroup: with the median bandwidth the cooperative kernel
+        selects the median in a workgroup of its own (False: in every row
+        workgroup; results are identical)."""
         self.L = _lib.load()
         self.cfg = dict(cfg)
         n = int(cfg["n"])
@@ -54,6 +57,10 @@ class Filter:
             rc = self.L.svpf_set_coop_shape(self.sp, int(coop_shape[0]), int(coop_shape[1]))
             if rc:
                 raise RuntimeError(f"svpf_set_coop_shape{tuple(coop_shape)} rejected ({rc})")
+        if not median_workgroup:
+            rc = self.L.svpf_set_coop_median_workgroup(self.sp, 0)
+            if rc:
+                raise RuntimeError(f"svpf_set_coop_median_workgroup rejected ({rc})")
 
     def path(self):
         """"single", "coop" or "grid": the kernel path of the next step."""

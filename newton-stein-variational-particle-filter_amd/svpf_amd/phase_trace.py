@@ -11,9 +11,12 @@ Stamps of tick t (slot t mod 256 of svpf_trace_buf, 64 per tick): 0 start,
 1 predict done, 2 predictive log-likelihood, 50 particles sorted, 3 bandwidth
 (median selection) done, 4 mixture means, then per Stein iteration k
 8 + 3k gradient done, 9 + 3k records gathered, 10 + 3k transport done, then
-40 outputs reduced, 41 final positions gathered, 42 partials gathered and the
-tick finished; inside the median selection 52 + r at the end of round r and 62
-when the rounds are done.
+40 outputs reduced, 41 final positions and partials gathered, 42 the tick
+finished; inside the median selection 52 + r at the end of round r and 62
+when the rounds are done.  With the median workgroup (the default for the
+median bandwidth) the sort and the selection run there: it stamps 44 when its
+predict is done and 45 when the median is published, and workgroup 0's wait for
+it falls into the first iteration's exchange.
 """
 from __future__ import annotations
 
@@ -30,16 +33,26 @@ TRACE_LIB = os.path.join(os.path.dirname(_lib.LIB_PATH), "libsvpf_amd_trace.so")
 CLOCK_HZ = 100e6
 
 
-def phase_split(buf: np.ndarray, ticks: range, iters: int) -> dict:
+def phase_split(buf: np.ndarray, ticks: range, iters: int, median_workgroup: bool = False) -> dict:
     """Median microseconds per phase over the given ticks; buf [256, 64] uint64 stamps."""
     us = lambda a, b: (buf[ticks.start:ticks.stop, b].astype(np.int64) - buf[ticks.start:ticks.stop, a].astype(np.int64)) / CLOCK_HZ * 1e6
-    out = {
-        "predict": us(0, 1),
-        "predictive_loglik": us(1, 2),
-        "sort": us(2, 50),
-        "median_selection": us(50, 3),
-        "mixture_means": us(3, 4),
-    }
+    if median_workgroup:
+        out = {
+            "predict": us(0, 1),
+            "predictive_loglik": us(1, 2),
+            "mixture_means": us(3, 4),
+            "median_workgroup_predict_done_after_tick_start": us(0, 44),
+            "median_workgroup_sort_and_selection": us(44, 45),
+            "median_published_after_tick_start": us(0, 45),
+        }
+    else:
+        out = {
+            "predict": us(0, 1),
+            "predictive_loglik": us(1, 2),
+            "sort": us(2, 50),
+            "median_selection": us(50, 3),
+            "mixture_means": us(3, 4),
+        }
     prev = 4
     grad, exch, tran = [], [], []
     for k in range(iters):
@@ -52,10 +65,12 @@ def phase_split(buf: np.ndarray, ticks: range, iters: int) -> dict:
     out["transport_per_iteration"] = np.mean(tran, axis=0)
     out["stein_loop_total"] = np.sum(grad, axis=0) + np.sum(exch, axis=0) + np.sum(tran, axis=0)
     out["outputs_reduction"] = us(prev, 40)
-    out["final_positions_exchange"] = us(40, 41)
-    out["partials_exchange_and_finish"] = us(41, 42)
+    out["final_exchange"] = us(40, 41)  # final positions and partials, one gather
+    out["finish"] = us(41, 42)
     out["tick"] = us(0, 42)
     res = {k: float(np.median(v)) for k, v in out.items()}
+    if median_workgroup:
+        return res
     # median selection: rounds (slots 52..61 stamped at the end of each) and the candidate phase
     sel = buf[ticks.start:ticks.stop]
     rounds = []
@@ -74,6 +89,8 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--ticks", type=int, default=200)
+    ap.add_argument("--median-workgroup", type=int, default=1, choices=(0, 1),
+                    help="1: the median in a workgroup of its own (the default); 0: in every row workgroup")
     args = ap.parse_args(argv)
     if not os.path.exists(TRACE_LIB):
         raise SystemExit(f"{TRACE_LIB} is not built: make -C newton-stein-variational-particle-filter_amd/csrc trace")
@@ -89,7 +106,7 @@ def main(argv=None):
     T = min(args.ticks, 256)
     cfg = config.plain_config(n=args.n)
     y, _ = data.sv_series(T, seed=0)
-    f = Filter(cfg)
+    f = Filter(cfg, median_workgroup=bool(args.median_workgroup))
     f.set_particles(data.initial_particles(args.n, -1.0, 0.97, 0.15, seed=2))
     path = f.path()
     if path != "coop":
@@ -101,9 +118,11 @@ def main(argv=None):
     if rc:
         raise SystemExit(f"svpf_trace_dump failed ({rc})")
     iters = int(cfg.get("stein_max_steps", 5))
-    split = phase_split(buf, range(10, T - 1), iters)  # skip the first ticks (the Stein budget's warm-up)
+    # skip the first ticks (the Stein budget's warm-up)
+    split = phase_split(buf, range(10, T - 1), iters, median_workgroup=bool(args.median_workgroup))
     f.close()
-    print(json.dumps({"n": args.n, "ticks": T, "stein_iterations": iters, "median_us": split}, indent=1))
+    print(json.dumps({"n": args.n, "ticks": T, "stein_iterations": iters, "median_workgroup": args.median_workgroup,
+                      "median_us": split}, indent=1))
 
 
 if __name__ == "__main__":

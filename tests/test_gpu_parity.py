@@ -252,6 +252,55 @@ def test_coop_shapes(torch_cuda, shape):
     assert np.max(np.abs(f.particles() - o.h)) <= 1e-4
 
 
+@pytest.mark.parametrize("name,n,shape", [("plain", 512, None), ("plain", 256, None), ("plain", 768, (8, 8)),
+                                          ("plain", 512, (8, 1)), ("median_guided", 512, None),
+                                          ("production_median", 384, None)])
+def test_coop_median_workgroup_is_bitwise_neutral(torch_cuda, name, n, shape):
+    """With the median bandwidth the cooperative kernel selects the median in a
+    workgroup of its own while the row workgroups compute the first gradient and
+    exchange its records unscaled.  Per-tick records, particles and RMSProp state
+    equal, bit for bit, those of the kernel whose every row workgroup selects the
+    median itself (per-tick steps and the device loop), and match the oracle."""
+    torch = torch_cuda
+    T = 12
+    if name == "plain":
+        cfg = config.plain_config(n=n)
+    else:
+        kw = dict(FEATURE_CONFIGS["median_guided" if name == "median_guided" else "production"])
+        kw["bandwidth_mode"] = 1
+        cfg = config.make_config(n=n, mu=-1.0, rho=0.97, sigma_z=0.15, gamma=-0.2, **kw)
+    y, _ = data.sv_series(T, seed=61)
+    rows = cfg.get("stein_max_steps", 5)
+    noise = data.predraw_noise(T, n, rows, seed=62, mim=True, rejuv=True)
+    h0 = data.initial_particles(n, -1.0, 0.97, 0.15, seed=63)
+    recs, parts, rms = [], [], []
+    for mw in (True, False):
+        print(f"coop median workgroup {mw}: {name} n={n} shape={shape}", flush=True)
+        f = Filter(cfg, path="coop", coop_shape=shape, median_workgroup=mw)
+        assert f.path() == "coop"
+        f.set_particles(h0)
+        tz = {k: dev(torch, noise.get(k)) for k in ("z_pred", "xi", "u_pred", "u_rej", "z_rej")}
+        f.set_noise(tz["z_pred"], tz["xi"], tz["u_pred"], tz["u_rej"], tz["z_rej"])
+        rec = np.zeros((T, 8), dtype=np.float32)
+        yp = 0.0
+        for t in range(T - 4):
+            f.step(float(y[t]), yp)
+            rec[t] = f.last_outputs()
+            yp = float(y[t])
+        ll = torch.zeros(4, device="cuda")
+        f.run_device(dev(torch, y[T - 4:]), ll)
+        rec[T - 4:, 0] = ll.cpu().numpy()
+        recs.append(rec)
+        parts.append(f.particles())
+        rms.append(f.rmsprop_state())
+        f.close()
+    assert np.array_equal(recs[0], recs[1])
+    assert np.array_equal(parts[0], parts[1])
+    assert np.array_equal(rms[0], rms[1])
+    o, out = run_oracle(cfg, y[: T - 4], {k: (v[: T - 4] if v is not None else None) for k, v in noise.items()}, h0)
+    assert np.sqrt(np.mean((recs[0][: T - 4, 2] - np.asarray(out["h_mean"])) ** 2)) <= 1e-4
+
+
 def test_coop_device_run_equals_per_tick(torch_cuda):
     torch = torch_cuda
     n, T = 512, 30
