@@ -587,7 +587,7 @@ def calibration_demo(torch, B=128, N=256, T=600, epochs=200, lr=1e-2):
             "recovery_limited_by": verdict, "wall_s": wall, "s_per_epoch": wall / epochs}
 
 
-def profile_only(torch, what, B, K, W):
+def profile_only(torch, what, B, K, W, n_large=16384):
     """One workload for a profiler run; prints what ran."""
     if what == "dominant":
         src = torch.ones(64 * 1024 * 1024, device="cuda")  # 256 MiB
@@ -598,7 +598,7 @@ def profile_only(torch, what, B, K, W):
         ms, flops, nf, groups = time_dominant_kernel(torch, cfgs, ys, h0s, n, W, K)
         log(f"copy 256 MiB; dominant group: {nf} filters, {K} ticks, {ms:.2f} ms, {flops / ms / 1e9:.2f} TFLOP/s")
     elif what == "large":
-        log(json.dumps(time_large(torch, 16384, T=K)))
+        log(json.dumps(time_large(torch, n_large, T=K)))
     elif what == "calib":
         print(json.dumps(calibration_demo(torch)), flush=True)
     elif what == "calib5":
@@ -627,6 +627,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--B", type=int, default=4096)
     ap.add_argument("--no-extras", action="store_true", help="headline only")
+    ap.add_argument("--n", type=int, default=16384, help="particles of --only large (default 16384)")
     ap.add_argument("--only", choices=["dominant", "diff", "single", "twofactor", "calib", "calib5", "large", "v2",
                                        "extended", "joint"],
                     default=None,
@@ -634,7 +635,7 @@ def main():
                          "batched workload preceded by a 256 MiB device copy (traffic calibration), the "
                          "differentiable filter, the single N=512 filter, the two-factor filter, the "
                          "two-factor calibration (prints its JSON; calib5: at B=1024, N=2048, T=1000 for "
-                         "--steps epochs), the full-feature N=16384 filter "
+                         "--steps epochs), the full-feature filter (N from --n, default 16384) "
                          "over --steps ticks, the fused single-workgroup step over --steps calls, or the "
                          "Heun / Student-t batched runs over --steps ticks")
     args = ap.parse_args()
@@ -652,7 +653,7 @@ def main():
     _lib.load()
     B, K, W = args.B, args.steps, args.warmup
     if args.only:
-        profile_only(torch, args.only, B, K, W)
+        profile_only(torch, args.only, B, K, W, n_large=args.n)
         return
     log(f"building workload B={B} T={W + K}")
     cfgs, ys, h0s, n = build_workload(B, W + K + 1)

@@ -480,6 +480,91 @@ SVPF_DEV void row_search(const float* s, int n, float si, int a, int b, const ui
     for (int q = 0; q < W; ++q) res[q] = pos[q] + 1;
 }
 
+// The value of rank need - 1 among M values, given C of them in cand[0, C) (LDS),
+// exactly those whose bit patterns lie in [L, H], with cL values below L.  With
+// bins (kHistBins words of LDS, free for this call; cand must not overlap them),
+// histogram passes -- kHistBins equal bins of bit patterns over [L, H], LDS
+// atomics, one prefix scan per wave -- narrow [L, H] to the bin holding the rank
+// until at most kMedianCands candidates remain, which are compacted into the
+// bins' words; the rank is then read off by counting (without bins C must be at
+// most kMedianCands).  ctr, res: one word of LDS each.  All threads call it.
+template <int BS>
+SVPF_DEV float select_candidates(const float* cand, int C, uint32_t L, uint32_t H, uint32_t cL, uint32_t need,
+                                 uint32_t* hist, uint32_t* ctr, uint32_t* res) {
+    const int lane = threadIdx.x & (kWave - 1);
+    if (L >= H) return __uint_as_float(H);
+    uint32_t cH = cL + (uint32_t)C;
+    if (hist && C > kMedianCands) {
+        const int C0 = C;
+        bool first = true;
+        while (cH - cL > (uint32_t)kMedianCands && L < H) {
+            const uint32_t span = H - L;  // bins cover [L, L + (kHistBins << sh)) >= [L, H]
+            int sh = 0;
+            while (((uint64_t)kHistBins << sh) <= (uint64_t)span) ++sh;
+            if (!first) __syncthreads();  // every wave has read the previous pass's bins
+            first = false;
+            for (int k = threadIdx.x; k < kHistBins; k += BS) hist[k] = 0u;
+            __syncthreads();
+            for (int k = threadIdx.x; k < C0; k += BS) {
+                const uint32_t x = __float_as_uint(cand[k]);
+                if (x >= L && x <= H) atomicAdd(&hist[(x - L) >> sh], 1u);
+            }
+            __syncthreads();
+            // every wave scans the bins (identical results): lane l holds bins 4l .. 4l+3
+            static_assert(kHistBins == 4 * kWave, "four bins per lane");
+            const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                           h3 = hist[4 * lane + 3];
+            const uint32_t incl = wave_scan_u32(h0 + h1 + h2 + h3);
+            const uint64_t hit = __ballot(cL + incl >= need);
+            // the bins hold every candidate of [L, H], so some lane reaches the rank
+            const int ls = __builtin_amdgcn_readfirstlane(hit ? (int)__builtin_ctzll(hit) : kWave - 1);
+            uint32_t base = cL + (uint32_t)__builtin_amdgcn_readlane((int)(incl - (h0 + h1 + h2 + h3)), ls);
+            const uint32_t g[4] = {(uint32_t)__builtin_amdgcn_readlane((int)h0, ls),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)h1, ls),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)h2, ls),
+                                   (uint32_t)__builtin_amdgcn_readlane((int)h3, ls)};
+            int b = 0;
+            while (b < 3 && base + g[b] < need) base += g[b++];
+            const uint32_t nL = L + ((uint32_t)(4 * ls + b) << sh);
+            const uint32_t top = nL + ((1u << sh) - 1u);
+            H = top < H ? top : H;
+            L = nL;
+            cL = base;
+            cH = base + g[b];
+        }
+        if (L >= H) return __uint_as_float(H);
+        // compact the candidates left in [L, H] into the bins' words
+        __syncthreads();
+        if (threadIdx.x == 0) *ctr = 0u;
+        __syncthreads();
+        float* small = reinterpret_cast<float*>(hist);
+        for (int k = threadIdx.x; k < C0; k += BS) {
+            const float v = cand[k];
+            const uint32_t x = __float_as_uint(v);
+            if (x >= L && x <= H) {
+                const uint32_t q = atomicAdd(ctr, 1u);
+                if (q < (uint32_t)kHistBins) small[q] = v;
+            }
+        }
+        __syncthreads();
+        cand = small;
+        C = (int)(cH - cL);
+    }
+    const uint32_t r0 = need - cL - 1u;
+    for (int k = threadIdx.x; k < C; k += BS) {
+        const float v = cand[k];
+        uint32_t below = 0u, upto = 0u;
+        for (int m = 0; m < C; ++m) {
+            const float u = cand[m];
+            below += (u < v) ? 1u : 0u;
+            upto += (u <= v) ? 1u : 0u;
+        }
+        if (below <= r0 && r0 < upto) *res = __float_as_uint(v);
+    }
+    __syncthreads();
+    return __uint_as_float(*res);
+}
+
 // Lower median (rank floor((M-1)/2), M = n(n-1)/2) of the pairwise squared
 // distances of the ascending array s[0, n), bit-identical to sorting all M
 // float32 values.  s needs room for n + 1 floats: s[n] is set to +inf.
@@ -657,78 +742,8 @@ SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi
         }
     }
     __syncthreads();
-    int C = (int)(cH - cL);
-    if (hsel && C > kMedianCands) {
-        // histogram passes over the candidates: kHistBins equal bins of bit patterns over
-        // [L, H]; the bin holding the rank becomes [L, H] until at most kMedianCands remain
-        const int C0 = C;
-        bool first = true;
-        while (cH - cL > (uint32_t)kMedianCands && L < H) {
-            const uint32_t span = H - L;  // bins cover [L, L + (kHistBins << sh)) >= [L, H]
-            int sh = 0;
-            while (((uint64_t)kHistBins << sh) <= (uint64_t)span) ++sh;
-            if (!first) __syncthreads();  // every wave has read the previous pass's bins
-            first = false;
-            for (int k = threadIdx.x; k < kHistBins; k += BS) hist[k] = 0u;
-            __syncthreads();
-            for (int k = threadIdx.x; k < C0; k += BS) {
-                const uint32_t x = __float_as_uint(cand[k]);
-                if (x >= L && x <= H) atomicAdd(&hist[(x - L) >> sh], 1u);
-            }
-            __syncthreads();
-            // every wave scans the bins (identical results): lane l holds bins 4l .. 4l+3
-            static_assert(kHistBins == 4 * kWave, "four bins per lane");
-            const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
-                           h3 = hist[4 * lane + 3];
-            const uint32_t incl = wave_scan_u32(h0 + h1 + h2 + h3);
-            const uint64_t hit = __ballot(cL + incl >= need);
-            // the bins hold every candidate of [L, H], so some lane reaches the rank
-            const int ls = __builtin_amdgcn_readfirstlane(hit ? (int)__builtin_ctzll(hit) : kWave - 1);
-            uint32_t base = cL + (uint32_t)__builtin_amdgcn_readlane((int)(incl - (h0 + h1 + h2 + h3)), ls);
-            const uint32_t g[4] = {(uint32_t)__builtin_amdgcn_readlane((int)h0, ls),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)h1, ls),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)h2, ls),
-                                   (uint32_t)__builtin_amdgcn_readlane((int)h3, ls)};
-            int b = 0;
-            while (b < 3 && base + g[b] < need) base += g[b++];
-            const uint32_t nL = L + ((uint32_t)(4 * ls + b) << sh);
-            const uint32_t top = nL + ((1u << sh) - 1u);
-            H = top < H ? top : H;
-            L = nL;
-            cL = base;
-            cH = base + g[b];
-        }
-        if (L >= H) return __uint_as_float(H);
-        // compact the candidates left in [L, H] into the bins' words
-        __syncthreads();
-        if (threadIdx.x == 0) red[0] = 0u;
-        __syncthreads();
-        float* small = reinterpret_cast<float*>(hist);
-        for (int k = threadIdx.x; k < C0; k += BS) {
-            const float v = cand[k];
-            const uint32_t x = __float_as_uint(v);
-            if (x >= L && x <= H) {
-                const uint32_t q = atomicAdd(&red[0], 1u);
-                if (q < (uint32_t)kHistBins) small[q] = v;
-            }
-        }
-        __syncthreads();
-        cand = small;
-        C = (int)(cH - cL);
-    }
-    const uint32_t r0 = need - cL - 1u;
-    for (int k = threadIdx.x; k < C; k += BS) {
-        const float v = cand[k];
-        uint32_t below = 0u, upto = 0u;
-        for (int m = 0; m < C; ++m) {
-            const float u = cand[m];
-            below += (u < v) ? 1u : 0u;
-            upto += (u <= v) ? 1u : 0u;
-        }
-        if (below <= r0 && r0 < upto) red[kResult] = __float_as_uint(v);
-    }
-    __syncthreads();
-    return __uint_as_float(red[kResult]);
+    const int C = (int)(cH - cL);
+    return select_candidates<BS>(cand, C, L, H, cL, need, hsel ? hist : nullptr, red, red + kResult);
 }
 
 SVPF_DEV float clamp_logvol(float h) { return fminf(fmaxf(h, -15.0f), 5.0f); }

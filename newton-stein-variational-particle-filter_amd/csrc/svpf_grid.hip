@@ -94,13 +94,12 @@ SVPF_DEV void iter_schedule(const svpf_fparams& P, const svpf_fstate& S, int bud
 // still inside [L, H], the first round seeded around the previous tick's value,
 // extended downwards / upwards 8x wider while the rank lies outside, then even
 // splits; here with 15 thresholds a round, one round per launch over (n-1)/256
-// workgroups of one row per lane, and up to 4096 candidates gathered at the end
-// and sorted.  Counts are exact integers, so the result is the float32 value of
+// workgroups of one row per lane, and up to kMedCand candidates gathered at the
+// end and narrowed by histogram passes (select_candidates, svpf_device.hpp).  Counts are exact integers, so the result is the float32 value of
 // rank floor((M-1)/2) among the M = n(n-1)/2 values fl(fl(s_j - s_i)^2).
 // ---------------------------------------------------------------------------
 constexpr int kMW = 15;            // thresholds per round
-constexpr int kMedCand = 1024;     // candidates gathered for the final selection
-constexpr int kMaxMedRounds = 5;   // round launches per tick (med_rounds); rare further rounds run in grid_median_final
+constexpr int kMedCand = 8192;     // candidates gathered for the final selection
 constexpr int kMedRows = 256;      // rows per round workgroup
 constexpr int kRun = 1024;         // sorted run length of the two-kernel sort
 constexpr int kMaxMG = (kMaxN + kMedRows - 1) / kMedRows;
@@ -114,7 +113,8 @@ struct MedState {
 };
 
 // round launches per tick for n particles: the rounds a seeded selection needs at that size
-int med_rounds(int n) { return n <= 1024 ? 2 : (n <= 4096 ? 3 : kMaxMedRounds); }
+// (16 even sub-brackets a round; the seeded first one holds about 1% of the pairs)
+int med_rounds(int n) { return n <= 1024 ? 1 : (n <= 4096 ? 2 : 3); }
 
 // dst[0, cnt) = src[0, cnt) with 16-byte loads, four in flight per thread (src, dst 16-byte aligned)
 template <int BS>
@@ -660,7 +660,8 @@ __global__ __launch_bounds__(1024) void grid_median_final(const svpf_fparams* __
     extern __shared__ __align__(16) char lds[];
     __shared__ uint32_t tot[kMW];
     __shared__ uint32_t wsum[BS / kWave];
-    __shared__ float medv;
+    __shared__ uint32_t bins[kHistBins];
+    __shared__ uint32_t ctr, medv;
     if (Pp->bandwidth_mode != SVPF_BW_MEDIAN) return;
     float* s = reinterpret_cast<float*>(lds);  // n + 1 floats
     float* cand = s + ((n + 4) & ~3);          // kMedCand floats
@@ -727,20 +728,7 @@ __global__ __launch_bounds__(1024) void grid_median_final(const svpf_fparams* __
             }
         }
         __syncthreads();
-        // the candidate of rank need - cL - 1 among the C: below <= rank < at-or-below
-        const uint32_t r0 = need - st.cL - 1u;
-        for (int k = threadIdx.x; k < C; k += BS) {
-            const float v = cand[k];
-            uint32_t below = 0u, upto = 0u;
-            for (int m = 0; m < C; ++m) {
-                const float u = cand[m];
-                below += (u < v) ? 1u : 0u;
-                upto += (u <= v) ? 1u : 0u;
-            }
-            if (below <= r0 && r0 < upto) medv = v;
-        }
-        __syncthreads();
-        med = medv;
+        med = select_candidates<BS>(cand, C, st.L, st.H, st.cL, need, bins, &ctr, &medv);
     }
     if (threadIdx.x == 0) {
         const float bw = fminf(fmaxf(sqrtf(med), 0.01f), 2.0f);
