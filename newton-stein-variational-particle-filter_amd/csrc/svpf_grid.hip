@@ -113,8 +113,11 @@ struct MedState {
 };
 
 // round launches per tick for n particles: the rounds a seeded selection needs at that size
-// (16 even sub-brackets a round; the seeded first one holds about 1% of the pairs)
-int med_rounds(int n) { return n <= 1024 ? 1 : (n <= 4096 ? 2 : 3); }
+// (16 even sub-brackets a round; the seeded first one holds about 1% of the pairs).
+// Above 4096 particles one launch more than the rounds need: the launch after the
+// last round applies its counts and gathers the candidates over the whole grid,
+// which grid_median_final, one workgroup, would otherwise do over every row.
+int med_rounds(int n) { return n <= 1024 ? 1 : (n <= 4096 ? 2 : 4); }
 
 // dst[0, cnt) = src[0, cnt) with 16-byte loads, four in flight per thread (src, dst 16-byte aligned)
 template <int BS>
@@ -536,22 +539,42 @@ __global__ __launch_bounds__(kGB) void grid_sort_merge(const svpf_fparams* __res
     if (i >= n) return;
     const int a = i / kRun, nr = (n + kRun - 1) / kRun;
     const float x = r[i];
-    int rank = i - a * kRun;
-    for (int b = 0; b < nr; ++b) {
-        if (b == a) continue;
-        const int lo = b * kRun, len = min(kRun, n - lo);
-        // elements of run b before x: y <= x (b < a) or y < x (b > a)
-        int pos = 0;
-        for (int step = kRun; step > 0; step >>= 1) {
-            const int q = pos + step;
-            if (q <= len) {
-                const float y = r[lo + q - 1];
-                pos = (b < a ? (y <= x) : (y < x)) ? q : pos;
-            }
-        }
-        rank += pos;
+    // elements of every other run b before x (y <= x for b < a, y < x for b > a): the
+    // binary searches of all runs advance in lockstep, their loads in flight together
+    constexpr int kMaxRuns = (kMaxN + kRun - 1) / kRun;
+    int pos[kMaxRuns], len[kMaxRuns];
+#pragma unroll
+    for (int b = 0; b < kMaxRuns; ++b) {
+        pos[b] = 0;
+        len[b] = (b < nr && b != a) ? min(kRun, n - b * kRun) : 0;
     }
+    for (int step = kRun; step > 0; step >>= 1) {
+        float y[kMaxRuns];
+#pragma unroll
+        for (int b = 0; b < kMaxRuns; ++b) {
+            const int q = pos[b] + step;
+            y[b] = r[q <= len[b] ? b * kRun + q - 1 : 0];
+        }
+#pragma unroll
+        for (int b = 0; b < kMaxRuns; ++b) {
+            const int q = pos[b] + step;
+            if (q <= len[b] && (b < a ? (y[b] <= x) : (y[b] < x))) pos[b] = q;
+        }
+    }
+    int rank = i - a * kRun;
+#pragma unroll
+    for (int b = 0; b < kMaxRuns; ++b) rank += pos[b];
     srt[rank] = x;
+}
+
+// tot[q] += sum over the nmg round workgroups of their count q (tot zeroed and
+// synchronised by the caller): every load in flight at once, LDS atomics.
+template <int BS>
+SVPF_DEV void med_sum_partials(const uint32_t* __restrict__ pp, int nmg, uint32_t* tot) {
+    for (int k = threadIdx.x; k < nmg * kMW; k += BS) {
+        const uint32_t c = pp[k];
+        if (c) atomicAdd(&tot[k % kMW], c);
+    }
 }
 
 // One selection round r of med_rounds(n): workgroup b owns rows [b*256, b*256+256).
@@ -579,12 +602,11 @@ __global__ __launch_bounds__(kMedRows) void grid_median_round(const svpf_fparams
     const int rows = n - 1;
     const int nmg = (rows + kMedRows - 1) / kMedRows;
     const int i = blockIdx.x * kMedRows + threadIdx.x;
-    if (r > 0 && threadIdx.x < kMW) {
-        // totals of the previous round, summed in a fixed order (identical in every workgroup)
-        const uint32_t* pp = mpart + (size_t)((r + 1) & 1) * kMaxMG * kMW;
-        uint32_t acc = 0u;
-        for (int b = 0; b < nmg; ++b) acc += pp[b * kMW + threadIdx.x];
-        tot[threadIdx.x] = acc;
+    if (r > 0) {
+        // totals of the previous round (integer counts: every workgroup gets the same sums)
+        if (threadIdx.x < kMW) tot[threadIdx.x] = 0u;
+        __syncthreads();
+        med_sum_partials<kMedRows>(mpart + (size_t)((r + 1) & 1) * kMaxMG * kMW, nmg, tot);
     }
     __syncthreads();
     if (r == 0) {
@@ -676,11 +698,9 @@ __global__ __launch_bounds__(1024) void grid_median_final(const svpf_fparams* __
     if (!st.done) {
         const uint32_t* pp = mpart + (size_t)((rounds - 1) & 1) * kMaxMG * kMW;
         const int nmg = (rows + kMedRows - 1) / kMedRows;
-        if (threadIdx.x < kMW) {
-            uint32_t acc = 0u;
-            for (int b = 0; b < nmg; ++b) acc += pp[b * kMW + threadIdx.x];
-            tot[threadIdx.x] = acc;
-        }
+        if (threadIdx.x < kMW) tot[threadIdx.x] = 0u;
+        __syncthreads();
+        med_sum_partials<BS>(pp, nmg, tot);
         __syncthreads();
         int qs;
         uint32_t xlo, xhi;
