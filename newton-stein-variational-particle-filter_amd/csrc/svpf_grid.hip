@@ -539,31 +539,21 @@ __global__ __launch_bounds__(kGB) void grid_sort_merge(const svpf_fparams* __res
     if (i >= n) return;
     const int a = i / kRun, nr = (n + kRun - 1) / kRun;
     const float x = r[i];
-    // elements of every other run b before x (y <= x for b < a, y < x for b > a): the
-    // binary searches of all runs advance in lockstep, their loads in flight together
-    constexpr int kMaxRuns = (kMaxN + kRun - 1) / kRun;
-    int pos[kMaxRuns], len[kMaxRuns];
-#pragma unroll
-    for (int b = 0; b < kMaxRuns; ++b) {
-        pos[b] = 0;
-        len[b] = (b < nr && b != a) ? min(kRun, n - b * kRun) : 0;
-    }
-    for (int step = kRun; step > 0; step >>= 1) {
-        float y[kMaxRuns];
-#pragma unroll
-        for (int b = 0; b < kMaxRuns; ++b) {
-            const int q = pos[b] + step;
-            y[b] = r[q <= len[b] ? b * kRun + q - 1 : 0];
-        }
-#pragma unroll
-        for (int b = 0; b < kMaxRuns; ++b) {
-            const int q = pos[b] + step;
-            if (q <= len[b] && (b < a ? (y[b] <= x) : (y[b] < x))) pos[b] = q;
-        }
-    }
     int rank = i - a * kRun;
-#pragma unroll
-    for (int b = 0; b < kMaxRuns; ++b) rank += pos[b];
+    for (int b = 0; b < nr; ++b) {
+        if (b == a) continue;
+        const int lo = b * kRun, len = min(kRun, n - lo);
+        // elements of run b before x: y <= x (b < a) or y < x (b > a)
+        int pos = 0;
+        for (int step = kRun; step > 0; step >>= 1) {
+            const int q = pos + step;
+            if (q <= len) {
+                const float y = r[lo + q - 1];
+                pos = (b < a ? (y <= x) : (y < x)) ? q : pos;
+            }
+        }
+        rank += pos;
+    }
     srt[rank] = x;
 }
 
