@@ -422,7 +422,7 @@ bool use_coop(const SVPFState* s) { return !use_grid(s) && coop_applies(s); }
 // workgroup being resident at once.
 bool coop_median_wg(const SVPFState* s) {
     const svpf_impl* I = s->impl;
-    return I->coop_bwg && s->bandwidth_mode == SVPF_BW_MEDIAN && I->coop_G + 1 <= 32;
+    return I->coop_bwg && s->bandwidth_mode == SVPF_BW_MEDIAN && I->coop_G <= 32;
 }
 
 // launch-sequence variants of the grid path

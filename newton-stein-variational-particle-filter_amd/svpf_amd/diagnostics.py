@@ -154,9 +154,11 @@ class Scores:
             self.g = None
 
     def __del__(self):
-        if _lib.finalizing():
-            return
+        # nothing to do while the interpreter shuts down (the runtime may be gone, and the
+        # binding module's globals with it)
         try:
+            if _lib.finalizing():
+                return
             self.close()
         except Exception:
             pass

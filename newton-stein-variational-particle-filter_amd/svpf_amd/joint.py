@@ -169,9 +169,14 @@ class JointFilter:
             self._s = None
 
     def __del__(self):
-        if _lib.finalizing():
-            return
-        self.close()
+        # nothing to do while the interpreter shuts down (the runtime may be gone, and the
+        # binding module's globals with it)
+        try:
+            if _lib.finalizing():
+                return
+            self.close()
+        except Exception:
+            pass
 
 
 def records_to_dicts(rec: np.ndarray) -> list[dict]:

@@ -54,9 +54,12 @@ class SVPF:
 
     def __del__(self):
         sp = getattr(self, "_sp", None)
-        if sp and not _lib.finalizing():
-            self._L.svpf_destroy(sp)
-            self._sp = None
+        try:
+            if sp and not _lib.finalizing():
+                self._L.svpf_destroy(sp)
+                self._sp = None
+        except Exception:  # the binding module is torn down: the process is exiting
+            pass
 
     @property
     def _st(self):
