@@ -417,9 +417,9 @@ bool use_grid(const SVPFState* s) {
 
 bool use_coop(const SVPFState* s) { return !use_grid(s) && coop_applies(s); }
 
-// The median workgroup runs beside the row workgroups when it fits beside them on one
-// XCD at one workgroup per CU (an XCD has 32 CUs): the plain launch relies on every
-// workgroup being resident at once.
+// The median workgroup runs beside at most 32 row workgroups: the plain launch relies on
+// every workgroup being resident at once, and 33 fit one XCD (32 CUs) as measured; beside 64
+// row workgroups the launch exceeded the XCD's capacity.
 bool coop_median_wg(const SVPFState* s) {
     const svpf_impl* I = s->impl;
     return I->coop_bwg && s->bandwidth_mode == SVPF_BW_MEDIAN && I->coop_G <= 32;
