@@ -1,9 +1,14 @@
 // svpf_batched.hpp -- the persistent batched kernel over tile shapes.
 //
-// A persistent workgroup pool pulls filters from an atomic work queue (filters
-// pre-sorted by decreasing N^2 cost) and advances each one through ticks
-// [t0, t1) with its particles resident in registers and LDS.  A single filter is
-// the B = 1 case.  EXT selects the instantiations with the Heun step and the
+// A persistent workgroup pool pulls work items from an atomic queue: item k is
+// tick t0 + k / nf of filter order[k % nf] (filters pre-sorted by decreasing N^2
+// cost), so the ticks of all filters interleave and the pool stays busy to the
+// end of the launch -- whole filters as items left up to a filter's T ticks of
+// idle tail.  An item loads the filter's particles into registers and LDS, runs
+// one tick and stores them; it first waits for the item of the filter's
+// previous tick (k - nf), which a running workgroup took earlier, through the
+// filter's progress word.  A single filter is the B = 1 case (no queue: the
+// workgroup advances it through all ticks).  EXT selects the instantiations with the Heun step and the
 // Student-t state; they live in their own translation unit (explicit
 // instantiation of batched_kernel<true>) so the common kernels keep their
 // register budget and the two sets compile in parallel.
@@ -39,7 +44,8 @@ __global__ __launch_bounds__(BS) void svpf_batched_kernel(const svpf_fparams* __
                                                           const svpf_fdesc* __restrict__ desc,
                                                           float* __restrict__ pool_h, float* __restrict__ pool_hp,
                                                           float* __restrict__ pool_v, const int* __restrict__ order,
-                                                          int nf, int* __restrict__ queue, int nmax, int t0, int t1) {
+                                                          int nf, int* __restrict__ queue, int* __restrict__ progress,
+                                                          int nmax, int t0, int t1) {
     extern __shared__ __align__(16) char lds[];
     __shared__ int next;
     if (queue == nullptr) {  // static assignment: workgroup b runs order[b]
@@ -48,13 +54,45 @@ __global__ __launch_bounds__(BS) void svpf_batched_kernel(const svpf_fparams* __
                                        t0, t1);
         return;
     }
+    constexpr long kSpinLimit = 1L << 24;  // polls of a wait for the previous tick (about 10 s)
+    const int items = nf * (t1 - t0);
+    // The item finished last: thread 0 publishes its filter's progress together with the
+    // next pop, after the barrier that ends the item (every thread's stores fenced).  A
+    // divergent store at the end of the loop body instead was merged with the pop across
+    // the back edge, which let lane 0 leave the loop while the other lanes repeated the item.
+    int done_slot = -1, done_ticks = 0;
     for (;;) {
-        if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+        if (threadIdx.x == 0) {
+            if (done_slot >= 0)
+                __hip_atomic_store(progress + done_slot, done_ticks, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            next = atomicAdd(queue, 1);
+        }
         __syncthreads();
-        const int k = next;
+        const int k = __builtin_amdgcn_readfirstlane(next);
         __syncthreads();
-        if (k >= nf) break;  // every workgroup reaches this once the queue is drained
-        run_filter<BS, R, CS, EXT>(lds, nmax, params, fstate, desc, order[k], pool_h, pool_hp, pool_v, t0, t1);
+        if (k >= items) break;  // every workgroup reaches this once the queue is drained
+        const int slot = k % nf, tt = k / nf;
+        if (tt > 0) {
+            // progress[slot] counts the filter's finished ticks of this launch; the item of
+            // its previous tick was taken before this one, by a workgroup that is running
+            if (threadIdx.x == 0) {
+                long spins = 0;
+                while (__hip_atomic_load(progress + slot, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < tt) {
+                    if (++spins > kSpinLimit) {
+                        fstate[order[slot]].status |= 2;  // reported by the host; the tick runs regardless
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __syncthreads();
+        }
+        run_filter<BS, R, CS, EXT>(lds, nmax, params, fstate, desc, order[slot], pool_h, pool_hp, pool_v, t0 + tt,
+                                   t0 + tt + 1);
+        __threadfence();  // this workgroup's stores of the filter's state reach L2 before the release
+        __syncthreads();
+        done_slot = slot;
+        done_ticks = tt + 1;
     }
 }
 

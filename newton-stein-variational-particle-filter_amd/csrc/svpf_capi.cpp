@@ -447,7 +447,7 @@ hipError_t launch_step_kernel(SVPFState* s, int t0, int t1, hipStream_t cs) {
                                 coop_median_wg(s) ? 1 : 0, cs);
     const int tile = I->tile | ((s->use_heun || s->use_student_t_state) ? SVPF_TILE_EXT : 0);
     return svpf_launch_batched(I->bs, tile, 1, I->dp(), I->ds(), I->dd(), I->d_h(), I->d_hp(), I->d_v(), I->dorder,
-                               1, nullptr, I->n, t0, t1, cs);
+                               1, nullptr, nullptr, I->n, t0, t1, cs);
 }
 
 hipError_t enqueue_tick(SVPFState* s, hipStream_t cs) {
@@ -1211,7 +1211,9 @@ int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, f
                                            static_cast<const svpf_fparams*>(d_params),
                                            static_cast<svpf_fstate*>(d_fstate), static_cast<const svpf_fdesc*>(d_desc),
                                            d_pool_h, d_pool_hp, d_pool_v, d_order + group_start[g], nf,
-                                           d_queues ? d_queues + g : nullptr, group_nmax[g], t0, t1, s);
+                                           d_queues ? d_queues + g : nullptr,
+                                           d_queues ? d_queues + n_groups + group_start[g] : nullptr, group_nmax[g],
+                                           t0, t1, s);
         if (e != hipSuccess) return SVPF_RC_HIP;
         if (fj) {
             if (hipEventRecord(fj->join[k], s) != hipSuccess) return SVPF_RC_HIP;

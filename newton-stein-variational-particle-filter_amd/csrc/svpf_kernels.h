@@ -34,7 +34,8 @@ int svpf_tile_capacity(int tile, int bs);
 int svpf_batched_wg_per_cu(int bs, int tile, int nmax);
 hipError_t svpf_launch_batched(int bs, int tile, int grid, const svpf_fparams* params, svpf_fstate* fstate,
                                const svpf_fdesc* desc, float* pool_h, float* pool_hp, float* pool_v,
-                               const int* order, int nf, int* queue, int nmax, int t0, int t1, hipStream_t stream);
+                               const int* order, int nf, int* queue, int* progress, int nmax, int t0, int t1,
+                               hipStream_t stream);
 
 #ifdef __cplusplus
 }

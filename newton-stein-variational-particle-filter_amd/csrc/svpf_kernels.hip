@@ -15,8 +15,10 @@ namespace svpf {
 template const void* batched_kernel<false>(int, int);
 }
 
-__global__ void svpf_queue_reset_kernel(int* queue) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *queue = 0;
+// The work queue's counter and the nf progress words of a batched launch start at 0.
+__global__ void svpf_queue_reset_kernel(int* queue, int* progress, int nf) {
+    if (threadIdx.x == 0) *queue = 0;
+    for (int i = threadIdx.x; i < nf; i += blockDim.x) progress[i] = 0;
 }
 
 // Lets every engine instance use up to 159 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
@@ -98,12 +100,14 @@ extern "C" int svpf_tile_capacity(int tile, int bs) {
 extern "C" hipError_t svpf_launch_batched(int bs, int tile, int grid, const svpf_fparams* params,
                                           svpf_fstate* fstate, const svpf_fdesc* desc, float* pool_h,
                                           float* pool_hp, float* pool_v, const int* order, int nf, int* queue,
-                                          int nmax, int t0, int t1, hipStream_t stream) {
+                                          int* progress, int nmax, int t0, int t1, hipStream_t stream) {
     const void* fn = batched_fn(bs, tile);
     if (!fn || grid <= 0) return hipErrorInvalidValue;
     size_t lds = LdsLayout::bytes(nmax);
-    if (queue) svpf_queue_reset_kernel<<<1, 64, 0, stream>>>(queue);
-    void* args[] = {&params, &fstate, &desc, &pool_h, &pool_hp, &pool_v, &order, &nf, &queue, &nmax, &t0, &t1};
+    if (queue && !progress) return hipErrorInvalidValue;
+    if (queue) svpf_queue_reset_kernel<<<1, 256, 0, stream>>>(queue, progress, nf);
+    void* args[] = {&params, &fstate, &desc, &pool_h, &pool_hp, &pool_v, &order, &nf, &queue, &progress, &nmax,
+                    &t0, &t1};
     hipError_t e = hipLaunchKernel(fn, dim3(grid), dim3(bs), args, lds, stream);
     if (e != hipSuccess) return e;
     return hipGetLastError();

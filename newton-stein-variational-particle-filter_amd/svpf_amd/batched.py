@@ -249,7 +249,8 @@ class Batch:
             slots = cus * per_cu if persistent_grid is None else persistent_grid
             ggrid.append(max(1, min(nf, slots)))
         self.order = torch.tensor(order, dtype=torch.int32, device=device)
-        self.queues = torch.zeros(max(1, len(keys)), dtype=torch.int32, device=device)
+        # per group its work-queue counter, then per filter (in order) its progress word
+        self.queues = torch.zeros(max(1, len(keys)) + len(order), dtype=torch.int32, device=device)
         self.groups = dict(start=np.array(gstart, np.int32), tile=np.array(gtile, np.int32),
                            bs=np.array(gbs, np.int32), grid=np.array(ggrid, np.int32),
                            nmax=np.array(gnmax, np.int32))
