@@ -376,7 +376,9 @@ static inline int svpf_get_fan_mode(const SVPFState* state) { return state->use_
  * group_bs[g] on grid group_grid[g] workgroups.  queues holds one int per
  * group, then one per filter (n_groups + group_start[n_groups] ints; the call
  * resets them): a group's work items are the ticks of its filters, interleaved,
- * and a filter's word counts its finished ticks.  A group holding a filter with the Heun step or the Student-t state
+ * and a filter's word counts its finished ticks (a tick whose wait for the
+ * previous one exceeds about a second of polling sets bit 1 of the filter's
+ * status).  A group holding a filter with the Heun step or the Student-t state
  * (SVPF_F_HEUN / SVPF_F_STUDENT_STATE) ORs SVPF_TILE_EXTENDED into its tile id;
  * without it such a filter is rejected (its svpf_fstate status is set). */
 int svpf_batched_run(const void* d_params, void* d_fstate, const void* d_desc, float* d_pool_h, float* d_pool_hp,

@@ -81,7 +81,8 @@ typedef struct svpf_fstate {
     int32_t timestep;
     int32_t guide_initialized;
     int32_t stein_steps_used;
-    int32_t status;            // non-zero: the kernel rejected this filter
+    int32_t status;            // bit 0: the kernel rejected this filter; bit 1: a tick's wait for
+                               // the filter's previous tick hit the spin limit
     float ksd_prev, vol_prev, h_mean_prev, guide_mean;
     float guide_var, bw_sq_ema, return_ema, return_var;
     float mu_state, mu_var, sigma_z_eff, y_prev;
