@@ -34,7 +34,8 @@ CLOCK_HZ = 100e6
 
 
 def phase_split(buf: np.ndarray, ticks: range, iters: int, median_workgroup: bool = False) -> dict:
-    """Median microseconds per phase over the given ticks; buf [256, 64] uint64 stamps."""
+    """Median microseconds per phase over the given ticks (and the mean and 90th percentile of
+    the tick); buf [256, 64] uint64 stamps."""
     us = lambda a, b: (buf[ticks.start:ticks.stop, b].astype(np.int64) - buf[ticks.start:ticks.stop, a].astype(np.int64)) / CLOCK_HZ * 1e6
     if median_workgroup:
         out = {
@@ -69,7 +70,10 @@ def phase_split(buf: np.ndarray, ticks: range, iters: int, median_workgroup: boo
     out["finish"] = us(41, 42)
     out["tick"] = us(0, 42)
     res = {k: float(np.median(v)) for k, v in out.items()}
+    res["tick_mean"] = float(np.mean(out["tick"]))
+    res["tick_p90"] = float(np.quantile(out["tick"], 0.9))
     if median_workgroup:
+        res["median_published_mean"] = float(np.mean(out["median_published_after_tick_start"]))
         return res
     # median selection: rounds (slots 52..61 stamped at the end of each) and the candidate phase
     sel = buf[ticks.start:ticks.stop]
