@@ -105,10 +105,13 @@ struct CoopEngine {
     }
     // Words [0, cnt) at addresses addr(k).  Eight loads per thread are in flight at
     // once, so an exchange of up to 8 BS words (the records of n = 512 with full
-    // Newton) costs one L2 round trip.
+    // Newton) costs one L2 round trip; each poll re-issues every word that is still
+    // stale together, so a late producer costs one round trip per poll, not one per
+    // word.
     template <class Addr, class Put>
     SVPF_DEV bool gather_at(int cnt, uint32_t epoch, Addr addr, Put put) {
         constexpr int U = 8;
+        const uint32_t want = epoch ^ epoch_xor;
         bool ok = true;
         for (int k0 = threadIdx.x; k0 < cnt; k0 += U * BS) {
             uint64_t w[U];
@@ -116,24 +119,30 @@ struct CoopEngine {
             for (int u = 0; u < U; ++u) {
                 int k = k0 + u * BS;
                 w[u] = k < cnt ? __hip_atomic_load(addr(k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                               : ((uint64_t)(epoch ^ epoch_xor) << 32);
+                               : ((uint64_t)want << 32);
             }
+            long spins = 0;
+            for (;;) {
+                bool stale = false;
+#pragma unroll
+                for (int u = 0; u < U; ++u) stale = stale || (uint32_t)(w[u] >> 32) != want;
+                if (!stale) break;
+                if (++spins > spin_limit) {
+                    ok = false;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if ((uint32_t)(w[u] >> 32) != want)  // k < cnt: padding words carry the epoch
+                        w[u] = __hip_atomic_load(addr(k0 + u * BS), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            if (!ok) break;
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 int k = k0 + u * BS;
-                long spins = 0;
-                while ((uint32_t)(w[u] >> 32) != (epoch ^ epoch_xor)) {
-                    if (++spins > spin_limit) {
-                        ok = false;
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    w[u] = __hip_atomic_load(addr(k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                if (!ok) break;
                 if (k < cnt) put(k, __uint_as_float((uint32_t)w[u]));
             }
-            if (!ok) break;
         }
         if (!ok) *flag = 1;
         __syncthreads();
