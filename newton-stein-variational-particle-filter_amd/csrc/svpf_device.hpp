@@ -395,6 +395,67 @@ SVPF_DEV void merge_level(const float* src, float* dst, int total, int lr) {
     }
 }
 
+// Half-cleaner steps J = 32 .. 1 of an ascending merge inside the wave (partners
+// at lane distance J; the lower lane keeps the minimum), for both registers.
+template <int J>
+SVPF_DEV void wave_clean_steps(float& x0, float& x1) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool lower = (lane & J) == 0;
+    const float y0 = xor_lane<J>(x0), y1 = xor_lane<J>(x1);
+    x0 = lower ? fminf(x0, y0) : fmaxf(x0, y0);
+    x1 = lower ? fminf(x1, y1) : fmaxf(x1, y1);
+    if constexpr (J > 1) wave_clean_steps<J / 2>(x0, x1);
+}
+
+// Stage 2 when the ps values fit the workgroup's registers (ps <= 2 BS, the
+// 128-runs sorted ascending in x0 = element e0 = 128 w + lane, x1 = e0 + 64):
+// bitonic merges in the form whose comparators all point one way -- for each
+// block size K = 256 .. ps, a flip step against the mirrored partner e ^ (K - 1),
+// then half-cleaners at e ^ J for J = K/4 .. 1 -- with the steps of J >= 128
+// through LDS (one barrier each, alternating dst and src) and the rest in
+// registers.  Writes the result to dst.
+template <int BS>
+SVPF_DEV void block_bitonic_tail(float x0, float x1, float* src, float* dst, int ps) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int e0 = 128 * w + lane, e1 = e0 + 64;
+    const bool act = e0 < ps;  // uniform per wave: ps is a multiple of 128
+    int stage = 0;
+    // partner e ^ mask; the lower element keeps the minimum
+    auto lds_step = [&](int mask) {
+        float* b = (stage & 1) ? src : dst;  // the first write goes to dst: src may still be read
+        ++stage;
+        if (act) {
+            b[e0] = x0;
+            b[e1] = x1;
+        }
+        __syncthreads();
+        if (act) {
+            const int p0 = e0 ^ mask, p1 = e1 ^ mask;
+            const float y0 = b[p0], y1 = b[p1];
+            const bool lo0 = e0 < p0, lo1 = e1 < p1;
+            x0 = lo0 ? fminf(x0, y0) : fmaxf(x0, y0);
+            x1 = lo1 ? fminf(x1, y1) : fmaxf(x1, y1);
+        }
+    };
+    for (int K = 256; K <= ps; K <<= 1) {
+        lds_step(K - 1);  // flip
+        for (int J = K / 4; J >= 128; J >>= 1) lds_step(J);
+        if (act) {
+            const float a = fminf(x0, x1), c = fmaxf(x0, x1);  // J = 64: e0 is the lower element
+            x0 = a;
+            x1 = c;
+            wave_clean_steps<32>(x0, x1);
+        }
+    }
+    // the last LDS step read from dst when its index was even: wait for every partner read
+    if (((stage - 1) & 1) == 0) __syncthreads();
+    if (act) {
+        dst[e0] = x0;
+        dst[e1] = x1;
+    }
+    __syncthreads();
+}
+
 // Sorts src[0, n) ascending into dst[0, n); src is overwritten.  Works over
 // the power-of-two domain ps = max(128, pow2_ceil(n)) with +inf padding, so
 // src and dst must hold ps floats when n > 128 (n floats otherwise); dst
@@ -409,6 +470,18 @@ SVPF_DEV void block_sort(float* src, float* dst, int n) {
     int ps = 128;
     while (ps < n) ps <<= 1;
     const int runs = ps / 128;
+    if (runs > 1 && ps <= 2 * BS) {
+        // every value in registers: wave-sorted runs, then bitonic merges
+        const int i0 = 128 * w + lane, i1 = i0 + 64;
+        float x0 = kInf, x1 = kInf;
+        if (i0 < ps) {
+            x0 = i0 < n ? src[i0] : kInf;
+            x1 = i1 < n ? src[i1] : kInf;
+            wave_sort128(x0, x1);
+        }
+        block_bitonic_tail<BS>(x0, x1, src, dst, ps);
+        return;
+    }
     const int wlim = runs == 1 ? n : ps;  // one run: no padding written
     int levels = 0;
     for (int r = runs; r > 4; r >>= 1) ++levels;
