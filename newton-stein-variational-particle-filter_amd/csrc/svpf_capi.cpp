@@ -149,12 +149,16 @@ ForkJoin* fork_join() {
 }
 
 // Cooperative shape for n particles: rows per workgroup = 256 / cs, G = ceil(n / rows).
-// Default: about 16 workgroups (rows between 8 and 256), all on one XCD (stride 8).
+// Default: about 16 workgroups (rows between 8 and 256), except that up to
+// n = 1024 a workgroup takes at most 32 rows (8 lanes each), up to 32 workgroups
+// (at n = 640 .. 1000 this measured 15-19% faster per tick than 16 workgroups of
+// 4 lanes per row); all on one XCD (stride 8).
 bool choose_coop_shape(svpf_impl* I, int cs, int stride) {
     const int n = I->n;
     if (cs <= 0) {
         int rows = 8;
         while (rows < 256 && rows * 16 < n) rows <<= 1;
+        if (n <= 1024 && rows > 32) rows = 32;
         cs = 256 / rows;
     }
     if (cs != 1 && cs != 2 && cs != 4 && cs != 8 && cs != 16 && cs != 32) return false;
