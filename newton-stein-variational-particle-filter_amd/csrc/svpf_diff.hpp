@@ -47,7 +47,8 @@ constexpr float kDiffAlpha = 0.8493218002880191f;  // sqrt(log2 e / 2)
 
 // Tape slot of one tick, in floats: x [n], u (predicted, before the clamp) [n],
 // then per iteration the TP_ROWS rows of n floats, then 4 scalars.
-enum { TP_H = 0, TP_V = 1, TP_PHI = 2, TP_EDD = 3, TP_VDD = 4, TP_CDD = 5, TP_L = 6, TP_UN = 7, TP_ROWS = 8 };
+// (The prior's third moment is not taped: its sigma adjoint follows from homogeneity, see tick_back.)
+enum { TP_H = 0, TP_V = 1, TP_PHI = 2, TP_EDD = 3, TP_VDD = 4, TP_L = 5, TP_UN = 6, TP_ROWS = 7 };
 __host__ __device__ inline size_t diff_slot_floats(int n, int S) { return (size_t)(2 + TP_ROWS * S) * n + 4; }
 
 struct DiffTheta {
@@ -167,12 +168,12 @@ struct DiffEngine {
 
     // ---- mixture-prior sums of own rows over all components (scaled units) ----
     template <bool MOM>
-    SVPF_DEV void prior_sums(const float (&hh)[R], float (&off)[R], float (&sr)[R], float (&s1)[R], float (&s2)[R],
-                             float (&s3)[R]) const {
+    SVPF_DEV void prior_sums(const float (&hh)[R], float (&off)[R], float (&sr)[R], float (&s1)[R],
+                             float (&s2)[R]) const {
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             off[r] = nearest_sq(hh[r]);
-            sr[r] = s1[r] = s2[r] = s3[r] = 0.f;
+            sr[r] = s1[r] = s2[r] = 0.f;
         }
         const int nq = (n + 3) >> 2;
         const float4* m4 = reinterpret_cast<const float4*>(mbuf);
@@ -188,11 +189,7 @@ struct DiffEngine {
                     const float ed = e * dd[c];
                     sr[r] += e;
                     s1[r] += ed;
-                    if (MOM) {
-                        const float edd = ed * dd[c];
-                        s2[r] += edd;
-                        s3[r] = fmaf(edd, dd[c], s3[r]);
-                    }
+                    if (MOM) s2[r] = fmaf(ed, dd[c], s2[r]);
                 }
             }
         };
@@ -360,10 +357,10 @@ struct DiffEngine {
             const float eff = 0.1f * sqrtf(beta);
             const float dstd = sqrtf(2.0f * eff * temp);
             __syncthreads();  // means complete; the previous transport finished reading rec
-            float hh[R], off[R], sr[R], s1[R], s2[R], s3[R];
+            float hh[R], off[R], sr[R], s1[R], s2[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) hh[r] = h[r] * sq;
-            prior_sums<RECORD>(hh, off, sr, s1, s2, s3);
+            prior_sums<RECORD>(hh, off, sr, s1, s2);
             float xs[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -387,7 +384,6 @@ struct DiffEngine {
                         it[TP_V * (size_t)n + i] = v[r];
                         it[TP_EDD * (size_t)n + i] = Edd;
                         it[TP_VDD * (size_t)n + i] = m2 - Edd * Edd;
-                        it[TP_CDD * (size_t)n + i] = s3[r] / sr[r] - Edd * m2;
                         it[TP_L * (size_t)n + i] = __log2f(sr[r]) - off[r];
                     }
                 }
@@ -502,6 +498,7 @@ struct DiffEngine {
         const float inv_s2 = 1.0f / (sigma * sigma);
         const float sq = sqrtf(kDiffLog2e * 0.5f * inv_s2);
         const float isq = inv_s2 / sq;
+        const float isq_m = 1.0f / sq, isig = 1.0f / sigma;
         const float sgn2 = C.paper_sign ? 2.0f : -2.0f;
         const float ysq = y_t * y_t;
         const float np1 = nu + 1.0f;
@@ -525,7 +522,7 @@ struct DiffEngine {
             const float* it = slot + 2 * (size_t)n + (size_t)k * TP_ROWS * n;
             const float beta = S > 1 ? 0.3f + 0.7f * (float)k / (float)(S - 1) : 1.0f;
             const float eff = 0.1f * sqrtf(beta);
-            float h[R], s[R], c[R], w[R], xs[R], curv[R], gmask[R], cmask[R], A[R], E[R], Edd[R], Vdd[R], Cdd[R], L[R];
+            float h[R], s[R], c[R], w[R], xs[R], curv[R], gmask[R], cmask[R], A[R], E[R], Edd[R], Vdd[R], L[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = row(r);
@@ -535,7 +532,6 @@ struct DiffEngine {
                 const float p = ok ? it[TP_PHI * (size_t)n + i] : 0.f;
                 Edd[r] = ok ? it[TP_EDD * (size_t)n + i] : 0.f;
                 Vdd[r] = ok ? it[TP_VDD * (size_t)n + i] : 0.f;
-                Cdd[r] = ok ? it[TP_CDD * (size_t)n + i] : 0.f;
                 L[r] = ok ? it[TP_L * (size_t)n + i] : 0.f;
                 const float un = ok ? it[TP_UN * (size_t)n + i] : 0.f;
                 E[r] = safe_expf(h[r]);
@@ -590,9 +586,12 @@ struct DiffEngine {
                 tn += (double)(glb * 0.5f * A[r] / opa + cvbar * 0.5f * A[r] / (opa * opa) - Abar * A[r] / nu);
                 ts += (double)(cvbar * (-2.0f * inv_s2 / sigma));
                 if (h[r] < 20.0f) hb[r] += Abar * (-A[r] * E[r] / (E[r] + 1e-8f));
-                hb[r] += gbar * inv_s2 * (kDiffKappa * Vdd[r] - 1.0f);
-                ts += (double)(gbar * (2.0f * Edd[r] / kDiffAlpha - Cdd[r] / (kDiffAlpha * kDiffAlpha * kDiffAlpha)) *
-                               inv_s2);
+                const float hp_ = gbar * inv_s2 * (kDiffKappa * Vdd[r] - 1.0f);
+                hb[r] += hp_;
+                // d gp / d sigma by homogeneity: scaling every particle, every mean and sigma by lambda scales
+                // gp by 1/lambda, so sigma d(gbar gp)/dsigma = -gbar gp - h hbar - sum_j m_j mbar_j with hbar,
+                // mbar the prior's position and mean adjoints (pass B); the means' part follows the Stein loop
+                ts -= (double)((gbar * (-Edd[r] * isq) + h[r] * hp_) * isig);
                 a[r] = gbar * inv_s2;
             }
 #pragma unroll
@@ -654,6 +653,7 @@ struct DiffEngine {
             tn += (double)(wt * (th.dst_const - 0.5f * log1pf(Ap[r]) + 0.5f * np1 * (Ap[r] / nu) / opa));
             // predict
             const float pm = (u[r] > -15.0f && u[r] < 5.0f) ? 1.0f : 0.0f;
+            ts -= (double)(mm[r] * isq_m * mb[r] * isig);  // the means' part of the prior's sigma adjoint
             const float mt = mb[r] + hb[r] * pm;
             ts += (double)(hb[r] * pm * noise_z(D, t, i));
             const float ex = safe_expf(xr[r] * 0.5f);

@@ -18,7 +18,7 @@
 // interval covers T), backward by segments, two O(N^2) adjoint passes per
 // Stein iteration:
 //   pass C: for each row, w_i . df/dd summed over j (row part) and the sums
-//     over j of K w_j, K^2 w_j d^T, K^2 w_j, K^3 w_j d d^T (column part);
+//     over j of K w_j, K^2 w_j d^T, K^2 w_j and K^3 d (w_j^T Qt_i d) (column part);
 //   pass B: for each component i, sum over j of rt_ij (a_j - psi_ij kappa sq * dd_ij).
 #pragma once
 
@@ -31,9 +31,11 @@ constexpr float kD2Clamp = 20.0f;
 
 // Tape slot of one tick (floats): x [2n], u [2n], then per iteration the
 // T2_ROWS rows of n floats, then 8 scalars (ell2_f, ell2_s, a_f, b_f, a_s, b_s).
+// The prior's third moments are not taped: the sigma adjoint of the mixture
+// prior follows from its homogeneity (see tick_back).
 enum {
     T2_HF = 0, T2_HS, T2_VF, T2_VS, T2_PF, T2_PS, T2_UF, T2_US, T2_EF, T2_ES,
-    T2_VFF, T2_VFS, T2_VSS, T2_WFF, T2_WFS, T2_WSF, T2_WSS, T2_L, T2_ROWS
+    T2_VFF, T2_VFS, T2_VSS, T2_L, T2_ROWS
 };
 __host__ __device__ inline size_t diff2d_slot_floats(int n, int S) { return (size_t)(4 + T2_ROWS * S) * n + 8; }
 
@@ -262,14 +264,13 @@ struct D2Engine {
             const float dstd = sqrtf(2.0f * eff * temp);
             __syncthreads();
             // mixture-prior sums (two components per float4 load)
-            float hhf[R], hhs[R], off[R], sr[R], s1f[R], s1s[R], mff[R], mfs[R], mss[R], cff[R], cfs[R], csf[R],
-                css[R];
+            float hhf[R], hhs[R], off[R], sr[R], s1f[R], s1s[R], mff[R], mfs[R], mss[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 hhf[r] = sqf * hf[r];
                 hhs[r] = sqs * hs[r];
                 off[r] = 3.0e38f;
-                sr[r] = s1f[r] = s1s[r] = mff[r] = mfs[r] = mss[r] = cff[r] = cfs[r] = csf[r] = css[r] = 0.f;
+                sr[r] = s1f[r] = s1s[r] = mff[r] = mfs[r] = mss[r] = 0.f;
             }
             const int nq = (n + 1) >> 1;
             const float4* m4 = reinterpret_cast<const float4*>(mbuf);
@@ -295,14 +296,9 @@ struct D2Engine {
                         s1f[r] += ef;
                         s1s[r] += es;
                         if (RECORD) {
-                            const float eff_ = ef * df[c], ess = es * ds[c];
-                            mff[r] += eff_;
+                            mff[r] = fmaf(ef, df[c], mff[r]);
                             mfs[r] = fmaf(ef, ds[c], mfs[r]);
-                            mss[r] += ess;
-                            cff[r] = fmaf(eff_, df[c], cff[r]);  // E[dd_f dd_f^2]
-                            cfs[r] = fmaf(ess, df[c], cfs[r]);   // E[dd_f dd_s^2]
-                            csf[r] = fmaf(eff_, ds[c], csf[r]);  // E[dd_s dd_f^2]
-                            css[r] = fmaf(ess, ds[c], css[r]);   // E[dd_s dd_s^2]
+                            mss[r] = fmaf(es, ds[c], mss[r]);
                         }
                     }
                 }
@@ -343,10 +339,6 @@ struct D2Engine {
                         it[T2_VFF * (size_t)n + i] = Mff - Ef * Ef;
                         it[T2_VFS * (size_t)n + i] = mfs[r] * isr - Ef * Es;
                         it[T2_VSS * (size_t)n + i] = Mss - Es * Es;
-                        it[T2_WFF * (size_t)n + i] = cff[r] * isr - Ef * Mff;
-                        it[T2_WFS * (size_t)n + i] = cfs[r] * isr - Ef * Mss;
-                        it[T2_WSF * (size_t)n + i] = csf[r] * isr - Es * Mff;
-                        it[T2_WSS * (size_t)n + i] = css[r] * isr - Es * Mss;
                         it[T2_L * (size_t)n + i] = __log2f(sr[r]) - off[r];
                     }
                 }
@@ -421,7 +413,36 @@ struct D2Engine {
         return pred_ll;
     }
 
-    // ---- backward tick.  (xbf, xbs, vbf, vbs): adjoints after the tick, replaced by those before ----
+    // score-side terms of one row at one Stein iteration, recomputed from the tape where needed: the
+    // likelihood terms, the clamped score and its mask, and the 2x2 Newton inverse [[ca, cb], [cb, cd]]
+    struct RowScore {
+        float A, E, hsum, gf, gs, mf, ms, ca, cb, cd;
+    };
+    SVPF_DEV static RowScore row_score(const svpf_d2config& C, const D2Theta& th, float ysq, float Dgf, float Dgs,
+                                       float sqf, float sqs, float beta, float hf, float hs, float Ef, float Es) {
+        RowScore q;
+        const float np1 = th.nu + 1.0f;
+        q.hsum = th.mu + hf + hs;
+        q.E = safe_expf(q.hsum);
+        q.A = ysq / (q.E + 1e-8f) / th.nu;
+        const float opa = 1.0f + q.A;
+        const float gl = -0.5f + 0.5f * np1 * q.A / opa - C.lik_offset;
+        const float grf = -Ef * (Dgf / sqf) + beta * gl, grs = -Es * (Dgs / sqs) + beta * gl;
+        q.mf = (grf > -10.0f && grf < 10.0f) ? 1.0f : 0.0f;
+        q.ms = (grs > -10.0f && grs < 10.0f) ? 1.0f : 0.0f;
+        q.gf = fminf(fmaxf(grf, -10.0f), 10.0f);
+        q.gs = fminf(fmaxf(grs, -10.0f), 10.0f);
+        const float cl = 0.5f * np1 * q.A / (opa * opa);
+        const float idet = 1.0f / (Dgf * Dgs + cl * (Dgf + Dgs));
+        q.ca = (Dgs + cl) * idet;
+        q.cb = -cl * idet;
+        q.cd = (Dgf + cl) * idet;
+        return q;
+    }
+
+    // ---- backward tick.  (xbf, xbs, vbf, vbs): adjoints after the tick, replaced by those before.
+    // Per-row values that pass C does not read are recomputed from the tape after it instead of being
+    // held in registers across it (one 1024-lane workgroup has 128 VGPRs a lane). ----
     SVPF_DEV void tick_back(const svpf_d2config& C, const D2Theta& th, const svpf_d2desc& D, int t, const float* slot,
                             float glbar, float (&xbf)[R], float (&xbs)[R], float (&vbf)[R], float (&vbs)[R],
                             double (&tb)[6]) {
@@ -433,27 +454,17 @@ struct D2Engine {
         if (threadIdx.x < 6) scal[threadIdx.x] = sc[threadIdx.x];
         __syncthreads();
         const float e2f = scal[0], e2s = scal[1];
-        int pr[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) pr[q] = __float_as_int(scal[2 + q]);
         const float rsf = 1.0f / fminf(fmaxf(sqrtf(e2f), 0.01f), 2.0f);
         const float rss = 1.0f / fminf(fmaxf(sqrtf(e2s), 0.01f), 2.0f);
         const float Dgf = 1.0f / (th.sig_f * th.sig_f), Dgs = 1.0f / (th.sig_s * th.sig_s);
         const float sqf = kDiffAlpha / th.sig_f, sqs = kDiffAlpha / th.sig_s;
         const float sgn2 = C.paper_sign ? 2.0f : -2.0f;
-        const float ia3 = 1.0f / (kDiffAlpha * kDiffAlpha * kDiffAlpha);
+        const float bwf = 1.0f / rsf, bws = 1.0f / rss, isgf = 1.0f / th.sig_f, isgs = 1.0f / th.sig_s;
 
-        float xrf[R], xrs[R], uuf[R], uus[R], mmf[R], mms[R], mbf[R], mbs[R], hbf[R], hbs[R], vbarf[R], vbars[R];
+        float mbf[R], mbs[R], hbf[R], hbs[R], vbarf[R], vbars[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const int i = row(r);
-            const bool ok = i < n;
-            xrf[r] = ok ? slot[i] : 0.f;
-            xrs[r] = ok ? slot[n + i] : 0.f;
-            uuf[r] = ok ? slot[2 * (size_t)n + i] : 0.f;
-            uus[r] = ok ? slot[3 * (size_t)n + i] : 0.f;
-            mmf[r] = sqf * th.rho_f * xrf[r];
-            mms[r] = sqs * th.rho_s * xrs[r];
+            const bool ok = valid(r);
             mbf[r] = mbs[r] = 0.f;
             hbf[r] = ok ? xbf[r] : 0.f;
             hbs[r] = ok ? xbs[r] : 0.f;
@@ -461,13 +472,12 @@ struct D2Engine {
             vbars[r] = ok ? vbs[r] : 0.f;
         }
         float rsbf = 0.f, rsbs = 0.f;
-        double tmu = 0.0, tsf = 0.0, tss = 0.0, tnu = 0.0;
+        float tmu = 0.f, tsf = 0.f, tss = 0.f, tnu = 0.f;
         for (int k = S - 1; k >= 0; --k) {
             const float* it = slot + 4 * (size_t)n + (size_t)k * T2_ROWS * n;
             const float beta = S > 1 ? 0.3f + 0.7f * (float)k / (float)(S - 1) : 1.0f;
             const float eff = 0.1f * sqrtf(beta);
-            float uf[R], us[R], sfv[R], ssv[R], gfv[R], gsv[R], mgf[R], mgs[R], ca[R], cb[R], cd[R], Av[R], Ev[R], hsum[R];
-            float wf[R], ws[R], Qff[R], Qfs[R], Qsf[R], Qss[R];
+            float uf[R], us[R], sfv[R], ssv[R], wf[R], ws[R], Qff[R], Qfs[R], Qsf[R], Qss[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = row(r);
@@ -475,46 +485,31 @@ struct D2Engine {
                 auto rd = [&](int row_) { return ok ? it[row_ * (size_t)n + i] : 0.f; };
                 const float hf = rd(T2_HF), hs = rd(T2_HS), vvf = rd(T2_VF), vvs = rd(T2_VS);
                 const float pf = rd(T2_PF), ps = rd(T2_PS), unf = rd(T2_UF), uns = rd(T2_US);
-                const float Ef = rd(T2_EF), Es = rd(T2_ES);
-                hsum[r] = th.mu + hf + hs;
-                Ev[r] = safe_expf(hsum[r]);
-                Av[r] = ysq / (Ev[r] + 1e-8f) / nu;
-                const float opa = 1.0f + Av[r];
-                const float gl = -0.5f + 0.5f * np1 * Av[r] / opa - C.lik_offset;
-                const float grf = -Ef * (Dgf / sqf) + beta * gl, grs = -Es * (Dgs / sqs) + beta * gl;
-                mgf[r] = (grf > -10.0f && grf < 10.0f) ? 1.0f : 0.0f;
-                mgs[r] = (grs > -10.0f && grs < 10.0f) ? 1.0f : 0.0f;
-                gfv[r] = fminf(fmaxf(grf, -10.0f), 10.0f);
-                gsv[r] = fminf(fmaxf(grs, -10.0f), 10.0f);
-                const float cl = 0.5f * np1 * Av[r] / (opa * opa);
-                const float idet = 1.0f / (Dgf * Dgs + cl * (Dgf + Dgs));
-                ca[r] = (Dgs + cl) * idet;
-                cb[r] = -cl * idet;
-                cd[r] = (Dgf + cl) * idet;
-                sfv[r] = 0.7f * (ca[r] * gfv[r] + cb[r] * gsv[r]);
-                ssv[r] = 0.7f * (cb[r] * gfv[r] + cd[r] * gsv[r]);
-                Qff[r] = sgn2 * ca[r] * rsf;
-                Qfs[r] = sgn2 * cb[r] * rss;
-                Qsf[r] = sgn2 * cb[r] * rsf;
-                Qss[r] = sgn2 * cd[r] * rss;
+                const RowScore q = row_score(C, th, ysq, Dgf, Dgs, sqf, sqs, beta, hf, hs, rd(T2_EF), rd(T2_ES));
+                sfv[r] = 0.7f * (q.ca * q.gf + q.cb * q.gs);
+                ssv[r] = 0.7f * (q.cb * q.gf + q.cd * q.gs);
+                Qff[r] = sgn2 * q.ca * rsf;
+                Qfs[r] = sgn2 * q.cb * rss;
+                Qsf[r] = sgn2 * q.cb * rsf;
+                Qss[r] = sgn2 * q.cd * rss;
                 // RMSProp + clamp, per coordinate
                 {
                     const float vn = C.rms_rho * vvf + (1.0f - C.rms_rho) * pf * pf;
-                    const float q = rsqrtf(vn + C.rms_eps);
+                    const float qq = rsqrtf(vn + C.rms_eps);
                     const float hm = (ok && unf > -kD2Clamp && unf < kD2Clamp) ? 1.0f : 0.0f;
                     const float hbb = hbf[r] * hm;
-                    const float vt = vbarf[r] + hbb * (-0.5f * eff * pf * q * q * q);
-                    wf[r] = ok ? (hbb * eff * q + vt * 2.0f * (1.0f - C.rms_rho) * pf) * inv_n : 0.f;
+                    const float vt = vbarf[r] + hbb * (-0.5f * eff * pf * qq * qq * qq);
+                    wf[r] = ok ? (hbb * eff * qq + vt * 2.0f * (1.0f - C.rms_rho) * pf) * inv_n : 0.f;
                     vbarf[r] = C.rms_rho * vt;
                     hbf[r] = hbb;
                 }
                 {
                     const float vn = C.rms_rho * vvs + (1.0f - C.rms_rho) * ps * ps;
-                    const float q = rsqrtf(vn + C.rms_eps);
+                    const float qq = rsqrtf(vn + C.rms_eps);
                     const float hm = (ok && uns > -kD2Clamp && uns < kD2Clamp) ? 1.0f : 0.0f;
                     const float hbb = hbs[r] * hm;
-                    const float vt = vbars[r] + hbb * (-0.5f * eff * ps * q * q * q);
-                    ws[r] = ok ? (hbb * eff * q + vt * 2.0f * (1.0f - C.rms_rho) * ps) * inv_n : 0.f;
+                    const float vt = vbars[r] + hbb * (-0.5f * eff * ps * qq * qq * qq);
+                    ws[r] = ok ? (hbb * eff * qq + vt * 2.0f * (1.0f - C.rms_rho) * ps) * inv_n : 0.f;
                     vbars[r] = C.rms_rho * vt;
                     hbs[r] = hbb;
                 }
@@ -526,18 +521,17 @@ struct D2Engine {
             for (int r = 0; r < R; ++r) {
                 if (!valid(r)) continue;
                 const int i = row(r);
-                rec[i] = make_float4(uf[r], us[r], sfv[r], ssv[r]);
+                rec[i] = make_float4(uf[r], us[r], -2.0f * sfv[r], -2.0f * ssv[r]);
                 rec2[i] = make_float4(Qff[r], Qfs[r], Qsf[r], Qss[r]);
                 rec3[i] = make_float4(wf[r], ws[r], 0.f, 0.f);
             }
             __syncthreads();
             // pass C
-            float Gf[R], Gs[R], C1f[R], C1s[R], QBff[R], QBfs[R], QBsf[R], QBss[R], KWf[R], KWs[R];
-            float Tfff[R], Tffs[R], Tfss[R], Tsff[R], Tsfs[R], Tsss[R];
+            float Gf[R], Gs[R], C1f[R], C1s[R], QBff[R], QBfs[R], QBsf[R], QBss[R], KWf[R], KWs[R], TCf[R], TCs[R];
 #pragma unroll
             for (int r = 0; r < R; ++r)
-                Gf[r] = Gs[r] = C1f[r] = C1s[r] = QBff[r] = QBfs[r] = QBsf[r] = QBss[r] = KWf[r] = KWs[r] = Tfff[r] =
-                    Tffs[r] = Tfss[r] = Tsff[r] = Tsfs[r] = Tsss[r] = 0.f;
+                Gf[r] = Gs[r] = C1f[r] = C1s[r] = QBff[r] = QBfs[r] = QBsf[r] = QBss[r] = KWf[r] = KWs[r] = TCf[r] =
+                    TCs[r] = 0.f;
             for (int j = 0; j < n; ++j) {
                 const float4 e = rec[j];
                 const float4 q = rec2[j];
@@ -547,32 +541,29 @@ struct D2Engine {
                     const float df = uf[r] - e.x, ds = us[r] - e.y;
                     const float K = __builtin_amdgcn_rcpf(fmaf(df, df, fmaf(ds, ds, 1.0f)));
                     const float K2 = K * K;
-                    // row part: w_i . df/dd with s_j, Qt_j
-                    const float al = fmaf(wf[r], e.z, ws[r] * e.w);
+                    // row part: w_i . df/dd with s_j (stored as -2 s_j), Qt_j:
+                    //   K^2 (b + (-4 K ga - 2 al) d), b = Qt_j^T w_i, ga = b . d, al = w_i . s_j
+                    const float al2 = fmaf(wf[r], e.z, ws[r] * e.w);
                     const float bf = fmaf(wf[r], q.x, ws[r] * q.z), bs = fmaf(wf[r], q.y, ws[r] * q.w);
                     const float ga = fmaf(bf, df, bs * ds);
-                    const float co = -2.0f * K2 * fmaf(2.0f * K, ga, al);
-                    Gf[r] = fmaf(co, df, fmaf(K2, bf, Gf[r]));
-                    Gs[r] = fmaf(co, ds, fmaf(K2, bs, Gs[r]));
+                    const float co = fmaf(-4.0f * K, ga, al2);
+                    Gf[r] = fmaf(K2, fmaf(co, df, bf), Gf[r]);
+                    Gs[r] = fmaf(K2, fmaf(co, ds, bs), Gs[r]);
                     // column part with the weights w_j
-                    const float kf = K * ww.x, ks = K * ww.y;
-                    C1f[r] += kf;
-                    C1s[r] += ks;
-                    const float k2f = K * kf, k2s = K * ks;
+                    C1f[r] = fmaf(K, ww.x, C1f[r]);
+                    C1s[r] = fmaf(K, ww.y, C1s[r]);
+                    const float k2f = K2 * ww.x, k2s = K2 * ww.y;
                     KWf[r] += k2f;
                     KWs[r] += k2s;
                     QBff[r] = fmaf(k2f, df, QBff[r]);
                     QBfs[r] = fmaf(k2f, ds, QBfs[r]);
                     QBsf[r] = fmaf(k2s, df, QBsf[r]);
                     QBss[r] = fmaf(k2s, ds, QBss[r]);
-                    const float k3f = K * k2f, k3s = K * k2s;
-                    const float dff = df * df, dfs = df * ds, dss = ds * ds;
-                    Tfff[r] = fmaf(k3f, dff, Tfff[r]);
-                    Tffs[r] = fmaf(k3f, dfs, Tffs[r]);
-                    Tfss[r] = fmaf(k3f, dss, Tfss[r]);
-                    Tsff[r] = fmaf(k3s, dff, Tsff[r]);
-                    Tsfs[r] = fmaf(k3s, dfs, Tsfs[r]);
-                    Tsss[r] = fmaf(k3s, dss, Tsss[r]);
+                    // sum_j K^3 w_j^T Qt_i d d: the third-order terms contracted with the row's own Qt
+                    const float Pf = fmaf(Qff[r], df, Qfs[r] * ds), Ps = fmaf(Qsf[r], df, Qss[r] * ds);
+                    const float k3p = K2 * K * fmaf(ww.x, Pf, ww.y * Ps);
+                    TCf[r] = fmaf(k3p, df, TCf[r]);
+                    TCs[r] = fmaf(k3p, ds, TCs[r]);
                 }
             }
             float atf[R], ats[R], aaf[R], aas[R], bb[R], Lv[R], Efv[R], Esv[R];
@@ -582,64 +573,66 @@ struct D2Engine {
                 if (!valid(r)) continue;  // rows past n see nonzero column sums: keep them out of the adjoints
                 const int i = row(r);
                 // column part of the position adjoint
-                const float colf = -(2.0f * (sfv[r] * QBff[r] + ssv[r] * QBsf[r]) -
-                                     4.0f * (Qff[r] * Tfff[r] + Qfs[r] * Tffs[r] + Qsf[r] * Tsff[r] + Qss[r] * Tsfs[r]) +
+                const float colf = -(2.0f * (sfv[r] * QBff[r] + ssv[r] * QBsf[r]) - 4.0f * TCf[r] +
                                      (KWf[r] * Qff[r] + KWs[r] * Qsf[r]));
-                const float cols = -(2.0f * (sfv[r] * QBfs[r] + ssv[r] * QBss[r]) -
-                                     4.0f * (Qff[r] * Tffs[r] + Qfs[r] * Tfss[r] + Qsf[r] * Tsfs[r] + Qss[r] * Tsss[r]) +
+                const float cols = -(2.0f * (sfv[r] * QBfs[r] + ssv[r] * QBss[r]) - 4.0f * TCs[r] +
                                      (KWf[r] * Qfs[r] + KWs[r] * Qss[r]));
                 const float Uf = Gf[r] + colf, Us = Gs[r] + cols;
                 hbf[r] += Uf * rsf;
                 hbs[r] += Us * rss;
                 rsbf += (uf[r] * Uf - (Qff[r] * QBff[r] + Qsf[r] * QBsf[r])) / rsf;
                 rsbs += (us[r] * Us - (Qfs[r] * QBfs[r] + Qss[r] * QBss[r])) / rss;
+                // the row's score terms again, from the tape
+                const float Ef = it[T2_EF * (size_t)n + i], Es = it[T2_ES * (size_t)n + i];
+                const RowScore q = row_score(C, th, ysq, Dgf, Dgs, sqf, sqs, beta, it[T2_HF * (size_t)n + i],
+                                             it[T2_HS * (size_t)n + i], Ef, Es);
                 // s = 0.7 Cinv g, Qt = sgn2 Cinv diag(rs)
                 const float sbf = C1f[r], sbs = C1s[r];
-                const float gbf = 0.7f * (ca[r] * sbf + cb[r] * sbs), gbs = 0.7f * (cb[r] * sbf + cd[r] * sbs);
+                const float gbf = 0.7f * (q.ca * sbf + q.cb * sbs), gbs = 0.7f * (q.cb * sbf + q.cd * sbs);
                 // Cinv adjoint (full 2x2): 0.7 sbar g^T + sgn2 Qbar, Qbar_ab = -QB_ab rs_b
-                const float Xff = 0.7f * sbf * gfv[r] + sgn2 * (-QBff[r] * rsf);
-                const float Xfs = 0.7f * sbf * gsv[r] + sgn2 * (-QBfs[r] * rss);
-                const float Xsf = 0.7f * sbs * gfv[r] + sgn2 * (-QBsf[r] * rsf);
-                const float Xss = 0.7f * sbs * gsv[r] + sgn2 * (-QBss[r] * rss);
+                const float Xff = 0.7f * sbf * q.gf + sgn2 * (-QBff[r] * rsf);
+                const float Xfs = 0.7f * sbf * q.gs + sgn2 * (-QBfs[r] * rss);
+                const float Xsf = 0.7f * sbs * q.gf + sgn2 * (-QBsf[r] * rsf);
+                const float Xss = 0.7f * sbs * q.gs + sgn2 * (-QBss[r] * rss);
                 // Cbar = -Cinv X Cinv (Cinv = [[ca, cb], [cb, cd]])
-                const float Yff = Xff * ca[r] + Xfs * cb[r], Yfs = Xff * cb[r] + Xfs * cd[r];
-                const float Ysf = Xsf * ca[r] + Xss * cb[r], Yss = Xsf * cb[r] + Xss * cd[r];
-                const float Cff = -(ca[r] * Yff + cb[r] * Ysf), Cfs = -(ca[r] * Yfs + cb[r] * Yss);
-                const float Csf = -(cb[r] * Yff + cd[r] * Ysf), Css = -(cb[r] * Yfs + cd[r] * Yss);
+                const float Yff = Xff * q.ca + Xfs * q.cb, Yfs = Xff * q.cb + Xfs * q.cd;
+                const float Ysf = Xsf * q.ca + Xss * q.cb, Yss = Xsf * q.cb + Xss * q.cd;
+                const float Cff = -(q.ca * Yff + q.cb * Ysf), Cfs = -(q.ca * Yfs + q.cb * Yss);
+                const float Csf = -(q.cb * Yff + q.cd * Ysf), Css = -(q.cb * Yfs + q.cd * Yss);
                 const float clb = Cff + Cfs + Csf + Css;
-                tsf += (double)(Cff * (-2.0f * Dgf / th.sig_f));
-                tss += (double)(Css * (-2.0f * Dgs / th.sig_s));
-                const float gpbf = gbf * mgf[r], gpbs = gbs * mgs[r];
+                tsf += Cff * (-2.0f * Dgf / th.sig_f);
+                tss += Css * (-2.0f * Dgs / th.sig_s);
+                const float gpbf = gbf * q.mf, gpbs = gbs * q.ms;
                 const float glb = beta * (gpbf + gpbs);
-                const float A = Av[r], opa = 1.0f + A;
+                const float A = q.A, opa = 1.0f + A;
                 const float Ab = glb * 0.5f * np1 / (opa * opa) + clb * 0.5f * np1 * (1.0f - A) / (opa * opa * opa);
-                tnu += (double)(glb * 0.5f * A / opa + clb * 0.5f * A / (opa * opa) - Ab * A / nu);
-                const float hbar_s = hsum[r] < 20.0f ? Ab * (-A * Ev[r] / (Ev[r] + 1e-8f)) : 0.f;
+                tnu += glb * 0.5f * A / opa + clb * 0.5f * A / (opa * opa) - Ab * A / nu;
+                const float hbar_s = q.hsum < 20.0f ? Ab * (-A * q.E / (q.E + 1e-8f)) : 0.f;
                 hbf[r] += hbar_s;
                 hbs[r] += hbar_s;
-                tmu += (double)hbar_s;
+                tmu += hbar_s;
                 // mixture prior of the own particle
-                const float* it2 = it;
-                const float Ef = it2[T2_EF * (size_t)n + i], Es = it2[T2_ES * (size_t)n + i];
-                const float Vff = it2[T2_VFF * (size_t)n + i], Vfs = it2[T2_VFS * (size_t)n + i];
-                const float Vss = it2[T2_VSS * (size_t)n + i];
-                const float Wff = it2[T2_WFF * (size_t)n + i], Wfs = it2[T2_WFS * (size_t)n + i];
-                const float Wsf = it2[T2_WSF * (size_t)n + i], Wss = it2[T2_WSS * (size_t)n + i];
+                const float Vff = it[T2_VFF * (size_t)n + i], Vfs = it[T2_VFS * (size_t)n + i];
+                const float Vss = it[T2_VSS * (size_t)n + i];
                 // d gp_c / d x_c' = -D_c (delta_cc' - V_cc' sq_c' / (sq_c alpha^2))
-                hbf[r] += -Dgf * gpbf * (1.0f - Vff * kDiffKappa) - Dgs * gpbs * (-Vfs * sqf / sqs * kDiffKappa);
-                hbs[r] += -Dgf * gpbf * (-Vfs * sqs / sqf * kDiffKappa) - Dgs * gpbs * (1.0f - Vss * kDiffKappa);
-                // d gp_c / d sigma_c''
-                tsf += (double)(gpbf * (2.0f * Ef / kDiffAlpha - Wff * ia3) * Dgf -
-                                gpbs * Wsf * ia3 / (th.sig_s * th.sig_f));
-                tss += (double)(gpbs * (2.0f * Es / kDiffAlpha - Wss * ia3) * Dgs -
-                                gpbf * Wfs * ia3 / (th.sig_f * th.sig_s));
+                const float Hf = -Dgf * gpbf * (1.0f - Vff * kDiffKappa) - Dgs * gpbs * (-Vfs * sqf / sqs * kDiffKappa);
+                const float Hs = -Dgf * gpbf * (-Vfs * sqs / sqf * kDiffKappa) - Dgs * gpbs * (1.0f - Vss * kDiffKappa);
+                hbf[r] += Hf;
+                hbs[r] += Hs;
+                // d gp / d sigma_c by homogeneity: scaling coordinate c of every particle, every mean and
+                // sigma_c by lambda scales gp_c by 1/lambda and leaves the other coordinate's gp as it is, so
+                //   sigma_c d(gpbar . gp)/dsigma_c = -gpbar_c gp_c - x_c xbar_c - sum_j m_jc mbar_jc
+                // with xbar, mbar the prior's position and mean adjoints (pass B's sums); the means' part is
+                // added once after the Stein loop, the means being fixed over it
+                tsf -= (gpbf * (-Ef * (Dgf / sqf)) + uf[r] * bwf * Hf) * isgf;
+                tss -= (gpbs * (-Es * (Dgs / sqs)) + us[r] * bws * Hs) * isgs;
                 // pass B record
                 aaf[r] = gpbf * Dgf;
                 aas[r] = gpbs * Dgs;
                 atf[r] = aaf[r] / sqf;
                 ats[r] = aas[r] / sqs;
                 bb[r] = atf[r] * Ef + ats[r] * Es;
-                Lv[r] = it2[T2_L * (size_t)n + i];
+                Lv[r] = it[T2_L * (size_t)n + i];
                 Efv[r] = Ef;
                 Esv[r] = Es;
             }
@@ -650,6 +643,14 @@ struct D2Engine {
                 const int i = row(r);
                 rec[i] = make_float4(uf[r] / rsf * sqf, us[r] / rss * sqs, Lv[r], bb[r]);
                 rec2[i] = make_float4(atf[r], ats[r], aaf[r], aas[r]);
+            }
+            // the scaled mixture means of the own rows (components), from the tick's start positions
+            float mmf[R], mms[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int i = row(r);
+                mmf[r] = i < n ? sqf * th.rho_f * slot[i] : 0.f;
+                mms[r] = i < n ? sqs * th.rho_s * slot[n + i] : 0.f;
             }
             __syncthreads();
             // pass B: components = own rows
@@ -675,6 +676,20 @@ struct D2Engine {
                 mbs[r] += accs[r];
             }
         }
+        // start positions and predicted particles of the tick, from the tape
+        float xrf[R], xrs[R], uuf[R], uus[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const int i = row(r);
+            const bool ok = i < n;
+            xrf[r] = ok ? slot[i] : 0.f;
+            xrs[r] = ok ? slot[n + i] : 0.f;
+            uuf[r] = ok ? slot[2 * (size_t)n + i] : 0.f;
+            uus[r] = ok ? slot[3 * (size_t)n + i] : 0.f;
+        }
+        int pr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) pr[q] = __float_as_int(scal[2 + q]);
         // bandwidths: ell_c^2 = (x_a - x_b)^2 of the predicted particles
         float hpf[R], hps[R];
 #pragma unroll
@@ -724,7 +739,7 @@ struct D2Engine {
             if (valid(r)) se[0] += expf(lw[r] - mx);
         block_sum<BS, 1>(se, red);
         const float inv_se = 1.0f / se[0];
-        double trf = 0.0, trs = 0.0;
+        float trf = 0.f, trs = 0.f;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const int i = row(r);
@@ -735,29 +750,33 @@ struct D2Engine {
             if (hh[r] < 20.0f) dh += 0.5f * np1 * Ap[r] / opa * (Ep[r] / (Ep[r] + 1e-8f));
             hbf[r] += wt * dh;
             hbs[r] += wt * dh;
-            tmu += (double)(wt * dh);
-            tnu += (double)(wt * (th.dst_const - 0.5f * log1pf(Ap[r]) + 0.5f * np1 * (Ap[r] / nu) / opa));
+            tmu += wt * dh;
+            tnu += wt * (th.dst_const - 0.5f * log1pf(Ap[r]) + 0.5f * np1 * (Ap[r] / nu) / opa);
             // predict
             float zf, zs;
             noise_z(D, t, i, zf, zs);
             const float pmf = (uuf[r] > -kD2Clamp && uuf[r] < kD2Clamp) ? 1.0f : 0.0f;
             const float pms = (uus[r] > -kD2Clamp && uus[r] < kD2Clamp) ? 1.0f : 0.0f;
+            // the means' part of the prior's sigma adjoint (homogeneity, above)
+            tsf -= th.rho_f * xrf[r] * mbf[r] * isgf;
+            tss -= th.rho_s * xrs[r] * mbs[r] * isgs;
             const float mtf = mbf[r] + hbf[r] * pmf, mts = mbs[r] + hbs[r] * pms;
-            tsf += (double)(hbf[r] * pmf * zf);
-            tss += (double)(hbs[r] * pms * zs);
-            trf += (double)(mtf * xrf[r]);
-            trs += (double)(mts * xrs[r]);
+            tsf += hbf[r] * pmf * zf;
+            tss += hbs[r] * pms * zs;
+            trf += mtf * xrf[r];
+            trs += mts * xrs[r];
             xbf[r] = mtf * th.rho_f;
             xbs[r] = mts * th.rho_s;
             vbf[r] = vbarf[r];
             vbs[r] = vbars[r];
         }
-        tb[0] += tmu;
-        tb[1] += trf;
-        tb[2] += tsf;
-        tb[3] += trs;
-        tb[4] += tss;
-        tb[5] += tnu;
+        // per-lane partials of one tick in float, summed over ticks in double
+        tb[0] += (double)tmu;
+        tb[1] += (double)trf;
+        tb[2] += (double)tsf;
+        tb[3] += (double)trs;
+        tb[4] += (double)tss;
+        tb[5] += (double)tnu;
     }
 };
 

@@ -159,8 +159,11 @@ def test_gradient_independent_of_checkpoint_interval(torch_cuda):
 
 
 def test_counter_generator_and_calibration_lowers_loss(torch_cuda):
+    # A well-conditioned generating theta: at sigma_s = 0.08, rho_s = 0.995 (THETAS[0]) the FP32
+    # gradient of a perturbed start differs from the FP64 one by up to 99% on some series (see
+    # GRAD_THETAS), so whether 25 Adam steps lower every filter's loss there turns on rounding.
     n, T, S = 128, 60, 5
-    true = THETAS[0]
+    true = GRAD_THETAS[0]
     y, x0, _, _ = inputs(n, T, S, [true] * 4, seed=11)
     bt = TwoFactorBatch(y, x0, n_stein=S, checkpoint_every=16)
     start = torch.tensor([true] * 4, device="cuda")
