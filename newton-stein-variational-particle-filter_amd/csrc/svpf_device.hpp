@@ -819,6 +819,80 @@ SVPF_DEV float median_pairwise_r2(float* s, int n, float est, Idx* plo, Idx* phi
     return select_candidates<BS>(cand, C, L, H, cL, need, hsel ? hist : nullptr, red, red + kResult);
 }
 
+// First pair (a, b), a < b, in row-major order (least a * n + b) of the n values vals[] whose
+// squared difference fl(fl(v_a - v_b)^2) equals r2 > 0 exactly; a = b = -1 when there is none.
+// x holds the calling thread's rows (row threadIdx.x + r BS), vals the same values by index and
+// sorted them ascending (both LDS, complete on entry).  On either side of a value x the squared
+// difference fl((x - v)^2) is monotone in v, so two binary searches over sorted tell whether x has
+// a partner; the rows that do are listed (list: n ints of LDS) and only pairs among them are
+// checked.  The result equals a scan of every pair, in O(n log n + m^2) for m listed rows.
+// words: 2 ints of LDS.  Every thread of the workgroup calls it.
+template <int BS, int R>
+SVPF_DEV void first_pair_at_r2(const float (&x)[R], const float* vals, const float* sorted, int n, float r2, int* list,
+                               int* words, int& a, int& b) {
+    if (threadIdx.x == 0) {
+        words[0] = 0;
+        words[1] = 0x7fffffff;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = (int)threadIdx.x + r * BS;
+        if (i >= n) continue;
+        const float xv = x[r];
+        // right side: the first sorted value above xv whose squared difference reaches r2
+        int lo = 0, hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const float v = sorted[mid];
+            const float d = xv - v;
+            if (v > xv && d * d >= r2) hi = mid;
+            else lo = mid + 1;
+        }
+        bool hit = false;
+        if (lo < n) {
+            const float d = xv - sorted[lo];
+            hit = d * d == r2;
+        }
+        // left side: the last sorted value below xv whose squared difference reaches r2
+        lo = 0;
+        hi = n;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const float v = sorted[mid];
+            const float d = xv - v;
+            if (v < xv && d * d >= r2) lo = mid + 1;
+            else hi = mid;
+        }
+        if (lo > 0) {
+            const float d = xv - sorted[lo - 1];
+            hit = hit || d * d == r2;
+        }
+        if (hit) list[atomicAdd(&words[0], 1)] = i;
+    }
+    __syncthreads();
+    const int m = words[0];
+    int best = 0x7fffffff;
+    for (int p = (int)threadIdx.x; p < m * m; p += BS) {
+        const int u = p / m, w = p - u * m;
+        const int i = list[u], j = list[w];
+        if (i < j) {
+            const float d = vals[i] - vals[j];
+            if (d * d == r2) best = min(best, i * n + j);
+        }
+    }
+    if (best != 0x7fffffff) atomicMin(&words[1], best);
+    __syncthreads();
+    const int k = words[1];
+    __syncthreads();
+    if (k == 0x7fffffff) {
+        a = b = -1;
+    } else {
+        a = k / n;
+        b = k - a * n;
+    }
+}
+
 SVPF_DEV float clamp_logvol(float h) { return fminf(fmaxf(h, -15.0f), 5.0f); }
 SVPF_DEV float safe_expf(float x) { return expf(fminf(x, 20.0f)); }
 

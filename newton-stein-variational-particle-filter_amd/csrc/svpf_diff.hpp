@@ -244,39 +244,23 @@ struct DiffEngine {
         return med;
     }
 
-    // ---- the pair (a, b), a < b, whose squared distance is ell2 (first in row-major order) ----
-    SVPF_DEV void median_pair(const float (&h)[R], float ell2, int& a, int& b) {
+    // ---- the pair (a, b), a < b, whose squared distance is ell2 (first in row-major order).
+    //      sorted: mbuf holds the values ascending (the median's sort); otherwise they are sorted here ----
+    SVPF_DEV void median_pair(const float (&h)[R], float ell2, bool sorted, int& a, int& b) {
+        if (!sorted) {
+            float* tmp = reinterpret_cast<float*>(rec);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (valid(r)) tmp[row(r)] = h[r];
+            __syncthreads();
+            block_sort<BS>(tmp, mbuf, n);
+        }
         float* hl = reinterpret_cast<float*>(recb);
-        int* key = reinterpret_cast<int*>(scal + 8);
-        if (threadIdx.x == 0) *key = 0x7fffffff;
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (valid(r)) hl[row(r)] = h[r];
-        __syncthreads();
-        int best = 0x7fffffff;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = row(r);
-            if (i >= n) continue;
-            const float hi = h[r];
-            for (int j = i + 1; j < n; ++j) {
-                const float d = hi - hl[j];
-                if (d * d == ell2) {
-                    best = min(best, i * n + j);
-                    break;
-                }
-            }
-        }
-        if (best != 0x7fffffff) atomicMin(key, best);
-        __syncthreads();
-        const int k = *key;
-        __syncthreads();
-        if (k == 0x7fffffff) {
-            a = b = -1;
-        } else {
-            a = k / n;
-            b = k - a * n;
-        }
+        first_pair_at_r2<BS, R>(h, hl, mbuf, n, ell2, reinterpret_cast<int*>(hl + n), reinterpret_cast<int*>(scal + 8),
+                                a, b);
     }
 
     // ---- one tick forward.  SELECT: select ell2 (the forward) or read the
@@ -319,7 +303,7 @@ struct DiffEngine {
             int a, b;
             const float bwr = sqrtf(ell2);
             if (bwr > 0.01f && bwr < 2.0f) {
-                median_pair(h, ell2, a, b);
+                median_pair(h, ell2, SELECT, a, b);
             } else {
                 a = b = -1;
             }

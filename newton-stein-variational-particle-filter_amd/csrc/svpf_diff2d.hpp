@@ -148,39 +148,23 @@ struct D2Engine {
         return med;
     }
 
-    // first pair (a, b), a < b, of one coordinate whose squared distance is ell2
-    SVPF_DEV void median_pair(const float (&h)[R], float ell2, int& a, int& b) {
+    // first pair (a, b), a < b, of one coordinate whose squared distance is ell2.
+    // sorted: mbuf holds the coordinate's values ascending (its median's sort); otherwise they are sorted here
+    SVPF_DEV void median_pair(const float (&h)[R], float ell2, bool sorted, int& a, int& b) {
+        if (!sorted) {
+            float* tmp = reinterpret_cast<float*>(rec);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                if (valid(r)) tmp[row(r)] = h[r];
+            __syncthreads();
+            block_sort<BS>(tmp, mbuf, n);
+        }
         float* hl = reinterpret_cast<float*>(rec3);
-        int* key = reinterpret_cast<int*>(scal + 8);
-        if (threadIdx.x == 0) *key = 0x7fffffff;
 #pragma unroll
         for (int r = 0; r < R; ++r)
             if (valid(r)) hl[row(r)] = h[r];
-        __syncthreads();
-        int best = 0x7fffffff;
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            const int i = row(r);
-            if (i >= n) continue;
-            const float hi = h[r];
-            for (int j = i + 1; j < n; ++j) {
-                const float d = hi - hl[j];
-                if (d * d == ell2) {
-                    best = min(best, i * n + j);
-                    break;
-                }
-            }
-        }
-        if (best != 0x7fffffff) atomicMin(key, best);
-        __syncthreads();
-        const int k = *key;
-        __syncthreads();
-        if (k == 0x7fffffff) {
-            a = b = -1;
-        } else {
-            a = k / n;
-            b = k - a * n;
-        }
+        first_pair_at_r2<BS, R>(h, hl, mbuf, n, ell2, reinterpret_cast<int*>(hl + n), reinterpret_cast<int*>(scal + 8),
+                                a, b);
     }
 
     // ---- forward tick ----
@@ -215,25 +199,23 @@ struct D2Engine {
         }
         const float pred_ll = logmeanexp(lw);
         float e2[2];
-        if (SELECT) {
-            e2[0] = median_coord(hf, med[0]);
-            e2[1] = median_coord(hs, med[1]);
-            med[0] = e2[0];
-            med[1] = e2[1];
-            if (threadIdx.x == 0) {
-                D.ell2[2 * t] = e2[0];
-                D.ell2[2 * t + 1] = e2[1];
+        int pr[4] = {-1, -1, -1, -1};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            // the pair search reads the coordinate's sorted values, so it follows its own median
+            if (SELECT) {
+                e2[c] = median_coord(c == 0 ? hf : hs, med[c]);
+                med[c] = e2[c];
+                if (threadIdx.x == 0) D.ell2[2 * t + c] = e2[c];
+            } else {
+                e2[c] = D.ell2[2 * t + c];
             }
-        } else {
-            e2[0] = D.ell2[2 * t];
-            e2[1] = D.ell2[2 * t + 1];
+            if (RECORD) {
+                const float b_ = sqrtf(e2[c]);
+                if (b_ > 0.01f && b_ < 2.0f) median_pair(c == 0 ? hf : hs, e2[c], SELECT, pr[2 * c], pr[2 * c + 1]);
+            }
         }
         if (RECORD) {
-            int pr[4] = {-1, -1, -1, -1};
-            for (int c = 0; c < 2; ++c) {
-                const float b_ = sqrtf(e2[c]);
-                if (b_ > 0.01f && b_ < 2.0f) median_pair(c == 0 ? hf : hs, e2[c], pr[2 * c], pr[2 * c + 1]);
-            }
             if (threadIdx.x == 0) {
                 float* sc = slot + (size_t)(4 + T2_ROWS * S) * n;
                 sc[0] = e2[0];
