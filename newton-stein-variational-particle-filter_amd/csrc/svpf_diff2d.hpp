@@ -635,11 +635,12 @@ struct D2Engine {
                 mms[r] = i < n ? sqs * th.rho_s * slot[n + i] : 0.f;
             }
             __syncthreads();
-            // pass B: components = own rows
+            // pass B: components = own rows; sum_j rt (a_j - psi kappa sq dd) with the constant
+            // kappa sq applied once after the sum
             const float kf = sqf * kDiffKappa, ks = sqs * kDiffKappa;
-            float accf[R], accs[R];
+            float accf[R], accs[R], apf[R], aps[R];
 #pragma unroll
-            for (int r = 0; r < R; ++r) accf[r] = accs[r] = 0.f;
+            for (int r = 0; r < R; ++r) accf[r] = accs[r] = apf[r] = aps[r] = 0.f;
             for (int j = 0; j < n; ++j) {
                 const float4 e = rec[j];
                 const float4 a = rec2[j];
@@ -647,15 +648,17 @@ struct D2Engine {
                 for (int r = 0; r < R; ++r) {
                     const float ddf = e.x - mmf[r], dds = e.y - mms[r];
                     const float rt = __builtin_amdgcn_exp2f(fmaf(-ddf, ddf, fmaf(-dds, dds, -e.z)));
-                    const float psi = fmaf(a.x, ddf, fmaf(a.y, dds, -e.w));
-                    accf[r] = fmaf(rt, fmaf(-psi * kf, ddf, a.z), accf[r]);
-                    accs[r] = fmaf(rt, fmaf(-psi * ks, dds, a.w), accs[r]);
+                    const float rp = rt * fmaf(a.x, ddf, fmaf(a.y, dds, -e.w));
+                    accf[r] = fmaf(rt, a.z, accf[r]);
+                    accs[r] = fmaf(rt, a.w, accs[r]);
+                    apf[r] = fmaf(rp, ddf, apf[r]);
+                    aps[r] = fmaf(rp, dds, aps[r]);
                 }
             }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
-                mbf[r] += accf[r];
-                mbs[r] += accs[r];
+                mbf[r] += fmaf(-kf, apf[r], accf[r]);
+                mbs[r] += fmaf(-ks, aps[r], accs[r]);
             }
         }
         // start positions and predicted particles of the tick, from the tape
