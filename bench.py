@@ -505,10 +505,11 @@ def twofactor_workload(B, N, T, seed=5):
     return ys, x0
 
 
-def time_twofactor(torch, B=1024, N=2048, T=32, C=32, drift_budget_gb=120.0):
+def time_twofactor(torch, B=1024, N=2048, T=32, C=32, drift_budget_gb=150.0):
     """Two-factor filter at the cfg shape: forward and forward + adjoint with checkpoints every C
-    ticks (the north star's 32; at T <= C the forward writes the whole tape), the Stein drift
-    kept for the backward when it fits drift_budget_gb (84 GB at T=1000: MI355X holds it).
+    ticks (32 for this workload; at T <= C the forward writes the whole tape), the Stein drift
+    and the prior's shift of every iteration kept for the backward when they fit drift_budget_gb
+    (126 GB at T=1000: MI355X holds it).
     Also the plain forward (checkpoints only, no drift, no tape) that the ratio is set against."""
     from svpf_amd.twofactor import TwoFactorBatch
 

@@ -28,6 +28,11 @@
 namespace svpf {
 
 constexpr float kD2Clamp = 20.0f;
+// Rows of the optional per-iteration buffer svpf_d2desc.drift ([T][S][kD2DriftRows][n]): the Stein drift
+// (phi_f, phi_s) and the mixture prior's shift (the nearest component's scaled squared distance), so that
+// the backward's recompute replays positions without the transport and the prior without its
+// nearest-component pass.
+constexpr int kD2DriftRows = 3;
 
 // Tape slot of one tick (floats): x [2n], u [2n], then per iteration the
 // T2_ROWS rows of n floats, then 8 scalars (ell2_f, ell2_s, a_f, b_f, a_s, b_s).
@@ -256,12 +261,24 @@ struct D2Engine {
             }
             const int nq = (n + 1) >> 1;
             const float4* m4 = reinterpret_cast<const float4*>(mbuf);
-            for (int q = 0; q < nq; ++q) {
-                const float4 mm = m4[q];
+            float* drift_k = D.drift ? D.drift + ((size_t)t * S + k) * kD2DriftRows * n : nullptr;
+            if (replay) {
+                // the forward's shift of the same positions (bit for bit what the pass below gives)
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const float a0 = hhf[r] - mm.x, b0 = hhs[r] - mm.y, a1 = hhf[r] - mm.z, b1 = hhs[r] - mm.w;
-                    off[r] = fminf(off[r], fminf(fmaf(a0, a0, b0 * b0), fmaf(a1, a1, b1 * b1)));
+                for (int r = 0; r < R; ++r) off[r] = valid(r) ? drift_k[2 * (size_t)n + row(r)] : 0.0f;
+            } else {
+                for (int q = 0; q < nq; ++q) {
+                    const float4 mm = m4[q];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        const float a0 = hhf[r] - mm.x, b0 = hhs[r] - mm.y, a1 = hhf[r] - mm.z, b1 = hhs[r] - mm.w;
+                        off[r] = fminf(off[r], fminf(fmaf(a0, a0, b0 * b0), fmaf(a1, a1, b1 * b1)));
+                    }
+                }
+                if (SELECT && drift_k) {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        if (valid(r)) drift_k[2 * (size_t)n + row(r)] = off[r];
                 }
             }
             for (int q = 0; q < nq; ++q) {
@@ -342,7 +359,6 @@ struct D2Engine {
                     as[r] = fmaf(K, fmaf(K, Ps, e.w), as[r]);
                 }
             }
-            float* drift_k = D.drift ? D.drift + ((size_t)t * S + k) * 2 * n : nullptr;
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = row(r);

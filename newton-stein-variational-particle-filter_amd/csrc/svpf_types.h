@@ -235,8 +235,9 @@ typedef struct svpf_d2desc {
     uint64_t seed;
     uint32_t filter_id;
     int32_t pad;
-    float* drift;         // [T][S][2][n] Stein drift of every iteration (forward output, optional): the
-                          // backward's recompute replays the positions from it instead of redoing the transport
+    float* drift;         // [T][S][3][n] per iteration the Stein drift (f, s) and the mixture prior's shift
+                          // (forward output, optional): the backward's recompute replays the positions from it
+                          // instead of redoing the transport, and skips the prior's nearest-component pass
     int32_t* mpair;       // [T][2][2] per coordinate the median pair (a, b) the adjoint differentiates
                           // through, -1 where that bandwidth is clamped (optional)
 } svpf_d2desc;

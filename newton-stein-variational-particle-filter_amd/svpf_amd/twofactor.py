@@ -86,10 +86,11 @@ class TwoFactorBatch:
         self.gtheta = torch.zeros((B, 6), **f32)
         self.gx0 = torch.zeros((B, 2, n), **f32)
         self.xT = torch.zeros((B, 2, n), **f32)
-        # the Stein drift of every forward iteration (the backward replays positions from it),
-        # kept when it fits the budget; otherwise the backward recomputes the transport
-        drift_gb = B * T * self.S * 2 * n * 4 / 1e9
-        self.drift = torch.empty((B, T, self.S, 2, n), **f32) if drift_gb <= drift_budget_gb else None
+        # the Stein drift and the prior's shift of every forward iteration (the backward replays
+        # positions from it and skips the prior's nearest-component pass), kept when it fits the
+        # budget; otherwise the backward recomputes the transport
+        drift_gb = B * T * self.S * 3 * n * 4 / 1e9
+        self.drift = torch.empty((B, T, self.S, 3, n), **f32) if drift_gb <= drift_budget_gb else None
         self.mpair = torch.full((B, T, 2, 2), -1, dtype=torch.int32, device=dev) if record_pairs else None
         self.noise = None
         if noise is not None:
@@ -121,7 +122,7 @@ class TwoFactorBatch:
             d["gx0"] = ptr(self.gx0, b, 2 * n)
             d["xT"] = ptr(self.xT, b, 2 * n)
             if self.drift is not None:
-                d["drift"] = ptr(self.drift, b, T * self.S * 2 * n)
+                d["drift"] = ptr(self.drift, b, T * self.S * 3 * n)
             if self.mpair is not None:
                 d["mpair"] = ptr(self.mpair, b, 4 * T)
             d["seed"] = seeds[b]
