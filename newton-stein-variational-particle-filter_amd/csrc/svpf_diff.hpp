@@ -279,8 +279,11 @@ struct DiffEngine {
             const float z = ok ? noise_z(D, t, i) : 0.f;
             const float xi_ = x[r];
             const float lev = C.gamma * y_prev / (safe_expf(xi_ * 0.5f) + 1e-8f);
-            m[r] = mu + rho * (xi_ - mu) + lev;
-            const float u = m[r] + sigma * z;
+            // explicit FMAs here and in the update below: the forward and the checkpoint recompute are
+            // two instantiations of this code, and contraction left to the compiler may round them apart
+            // (the recompute must reproduce the forward's positions bit for bit)
+            m[r] = fmaf(rho, xi_ - mu, mu) + lev;
+            const float u = fmaf(sigma, z, m[r]);
             h[r] = clamp_logvol(u);
             if (RECORD && ok) {
                 slot[i] = xi_;
@@ -383,10 +386,9 @@ struct DiffEngine {
                 const int i = row(r);
                 const float p = replay ? (i < n ? drift_k[i] : 0.0f) : acc[r] * inv_n;
                 if (SELECT && drift_k && i < n) drift_k[i] = p;
-                const float vn = C.rms_rho * v[r] + (1.0f - C.rms_rho) * p * p;
-                const float drift = eff * p * rsqrtf(vn + C.rms_eps);
+                const float vn = fmaf(C.rms_rho, v[r], (1.0f - C.rms_rho) * (p * p));
                 const float xi = (temp > 1e-6f && i < n) ? noise_xi(D, S, t, k, i) : 0.0f;
-                const float un = h[r] + drift + dstd * xi;
+                const float un = fmaf(dstd, xi, fmaf(eff * p, rsqrtf(vn + C.rms_eps), h[r]));
                 if (RECORD && i < n) {
                     it[TP_PHI * (size_t)n + i] = p;
                     it[TP_UN * (size_t)n + i] = un;
@@ -495,7 +497,7 @@ struct DiffEngine {
             xr[r] = ok ? slot[i] : 0.f;
             u[r] = ok ? slot[n + i] : 0.f;
             const float lev = C.gamma * y_prev / (safe_expf(xr[r] * 0.5f) + 1e-8f);
-            mm[r] = sq * (mu + rho * (xr[r] - mu) + lev);
+            mm[r] = sq * (fmaf(rho, xr[r] - mu, mu) + lev);  // the forward's means, rounded alike
             mb[r] = 0.f;
             hb[r] = ok ? xb[r] : 0.f;
             vbar[r] = ok ? vb[r] : 0.f;
@@ -532,7 +534,7 @@ struct DiffEngine {
                 s[r] = 0.7f * g / curv[r];
                 c[r] = sgn2 / curv[r];
                 // RMSProp + clamp
-                const float vn = C.rms_rho * vv + (1.0f - C.rms_rho) * p * p;
+                const float vn = fmaf(C.rms_rho, vv, (1.0f - C.rms_rho) * (p * p));
                 const float q = rsqrtf(vn + C.rms_eps);
                 const float hm = (ok && un > -15.0f && un < 5.0f) ? 1.0f : 0.0f;
                 const float hbb = hb[r] * hm;

@@ -156,20 +156,33 @@ def test_adjoint_matches_oracle_central_differences(torch_cuda):
     assert rel_err(g, fd) <= 1e-3, (g, fd)
 
 
-def test_gradient_independent_of_checkpoint_interval(torch_cuda):
-    """Bit-identical gradients for every checkpoint interval, with the backward
-    replaying positions from the stored Stein drift or (no drift buffer)
-    recomputing the transport."""
-    n, T, S = 128, 9, 5
-    grads = []
-    for C, budget in ((1, 32.0), (2, 32.0), (4, 32.0), (9, 32.0), (2, 0.0), (4, 0.0)):
-        bt, _ = make_batch(n, T, S, GRAD_THETAS, C, seed=7, drift_budget_gb=budget)
+@pytest.mark.parametrize("n,T,runs", [
+    (128, 9, ((1, 32.0), (2, 32.0), (4, 32.0), (9, 32.0), (2, 0.0), (4, 0.0))),
+    (2048, 12, ((12, 32.0), (4, 32.0), (4, 0.0))),  # two rows per lane
+])
+def test_gradient_independent_of_checkpoint_interval(torch_cuda, n, T, runs):
+    """Bit-identical gradients and median pairs for every checkpoint interval, with the
+    backward replaying positions from the stored Stein drift or (no drift buffer)
+    recomputing the transport: the recompute reproduces the forward's positions bit for
+    bit, so the pair the forward's bandwidth came from is found again at every tick."""
+    S = 5
+    grads, pairs = [], []
+    for C, budget in runs:
+        y, h0, z, xi = inputs(n, T, S, GRAD_THETAS, seed=7)
+        bt = DiffBatch(y, h0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi},
+                       drift_budget_gb=budget, record_pairs=True)
         assert (bt.drift is None) == (budget == 0.0)
         theta = torch.tensor(GRAD_THETAS, device="cuda", requires_grad=True)
         (-predictive_loglik(theta, bt).sum()).backward()
         grads.append(theta.grad.cpu().numpy())
-    for g in grads[1:]:
+        pairs.append(bt.mpair.cpu().numpy())
+        bw = np.sqrt(bt.ell2.cpu().numpy())
+        inside = (bw > 0.01) & (bw < 2.0)
+        assert (pairs[-1][:, :, 0][inside] >= 0).all(), (C, budget)
+        del bt
+    for g, p in zip(grads[1:], pairs[1:]):
         assert np.array_equal(g, grads[0])
+        assert np.array_equal(p, pairs[0])
 
 
 def test_counter_generator_forward_and_backward(torch_cuda):

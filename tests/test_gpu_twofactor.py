@@ -143,19 +143,32 @@ def test_adjoint_at_the_cfg5_particle_count(torch_cuda):
     assert rel_err(g, ref) <= 1e-3, (g, ref)
 
 
-def test_gradient_independent_of_checkpoint_interval(torch_cuda):
-    """Bit-identical gradients for every checkpoint interval, replaying the
-    stored Stein drift or (no drift buffer) recomputing the transport."""
-    n, T, S = 128, 7, 5
-    grads = []
-    for C, budget in ((1, 32.0), (3, 32.0), (7, 32.0), (3, 0.0)):
-        bt, _ = make(n, T, S, GRAD_THETAS, C, seed=7, drift_budget_gb=budget)
+@pytest.mark.parametrize("n,T,runs", [
+    (128, 7, ((1, 32.0), (3, 32.0), (7, 32.0), (3, 0.0))),
+    (2048, 9, ((9, 32.0), (3, 32.0), (3, 0.0))),  # two rows per lane
+])
+def test_gradient_independent_of_checkpoint_interval(torch_cuda, n, T, runs):
+    """Bit-identical gradients and median pairs for every checkpoint interval, replaying
+    the stored Stein drift and prior shift or (no such buffer) recomputing the transport
+    and the nearest-component pass."""
+    S = 5
+    grads, pairs = [], []
+    for C, budget in runs:
+        y, x0, z, xi = inputs(n, T, S, GRAD_THETAS, seed=7)
+        bt = TwoFactorBatch(y, x0, n_stein=S, checkpoint_every=C, noise={"z": z, "xi": xi},
+                            drift_budget_gb=budget, record_pairs=True)
         assert (bt.drift is None) == (budget == 0.0)
         theta = torch.tensor(GRAD_THETAS, device="cuda", requires_grad=True)
         (-predictive_loglik_2d(theta, bt).sum()).backward()
         grads.append(theta.grad.cpu().numpy())
-    for g in grads[1:]:
+        pairs.append(bt.mpair.cpu().numpy())
+        bw = np.sqrt(bt.ell2.cpu().numpy())
+        inside = (bw > 0.01) & (bw < 2.0)
+        assert (pairs[-1][:, :, :, 0][inside] >= 0).all(), (C, budget)
+        del bt
+    for g, p in zip(grads[1:], pairs[1:]):
         assert np.array_equal(g, grads[0])
+        assert np.array_equal(p, pairs[0])
 
 
 def test_counter_generator_and_calibration_lowers_loss(torch_cuda):
