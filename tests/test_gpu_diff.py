@@ -158,6 +158,8 @@ def test_adjoint_matches_oracle_central_differences(torch_cuda):
 
 @pytest.mark.parametrize("n,T,runs", [
     (128, 9, ((1, 32.0), (2, 32.0), (4, 32.0), (9, 32.0), (2, 0.0), (4, 0.0))),
+    (300, 8, ((8, 32.0), (3, 32.0), (3, 0.0))),  # 512-thread workgroups
+    (1000, 8, ((8, 32.0), (3, 32.0), (3, 0.0))),  # 1024-thread workgroups, one row per lane
     (2048, 12, ((12, 32.0), (4, 32.0), (4, 0.0))),  # two rows per lane
 ])
 def test_gradient_independent_of_checkpoint_interval(torch_cuda, n, T, runs):

@@ -145,6 +145,8 @@ def test_adjoint_at_the_cfg5_particle_count(torch_cuda):
 
 @pytest.mark.parametrize("n,T,runs", [
     (128, 7, ((1, 32.0), (3, 32.0), (7, 32.0), (3, 0.0))),
+    (300, 6, ((6, 32.0), (2, 32.0), (2, 0.0))),  # 512-thread workgroups
+    (1000, 6, ((6, 32.0), (2, 32.0), (2, 0.0))),  # 1024-thread workgroups, one row per lane
     (2048, 9, ((9, 32.0), (3, 32.0), (3, 0.0))),  # two rows per lane
 ])
 def test_gradient_independent_of_checkpoint_interval(torch_cuda, n, T, runs):
