@@ -189,9 +189,7 @@ struct D2Engine {
             if (ok) noise_z(D, t, i, zf, zs);
             mf[r] = th.rho_f * xf[r];
             ms[r] = th.rho_s * xs[r];
-            // explicit FMAs here and in the update below: the forward and the checkpoint recompute are
-            // two instantiations of this code, and contraction left to the compiler may round them apart
-            const float uf = fmaf(th.sig_f, zf, mf[r]), us = fmaf(th.sig_s, zs, ms[r]);
+            const float uf = mf[r] + th.sig_f * zf, us = ms[r] + th.sig_s * zs;
             hf[r] = fminf(fmaxf(uf, -kD2Clamp), kD2Clamp);
             hs[r] = fminf(fmaxf(us, -kD2Clamp), kD2Clamp);
             if (RECORD && ok) {
@@ -372,12 +370,12 @@ struct D2Engine {
                     drift_k[i] = pf;
                     drift_k[n + i] = ps;
                 }
-                const float vnf = fmaf(C.rms_rho, vf[r], (1.0f - C.rms_rho) * (pf * pf));
-                const float vns = fmaf(C.rms_rho, vs[r], (1.0f - C.rms_rho) * (ps * ps));
+                const float vnf = C.rms_rho * vf[r] + (1.0f - C.rms_rho) * pf * pf;
+                const float vns = C.rms_rho * vs[r] + (1.0f - C.rms_rho) * ps * ps;
                 float xa = 0.f, xb = 0.f;
                 if (temp > 1e-6f && i < n) noise_xi(D, S, t, k, i, xa, xb);
-                const float unf = fmaf(dstd, xa, fmaf(eff * pf, rsqrtf(vnf + C.rms_eps), hf[r]));
-                const float uns = fmaf(dstd, xb, fmaf(eff * ps, rsqrtf(vns + C.rms_eps), hs[r]));
+                const float unf = hf[r] + eff * pf * rsqrtf(vnf + C.rms_eps) + dstd * xa;
+                const float uns = hs[r] + eff * ps * rsqrtf(vns + C.rms_eps) + dstd * xb;
                 if (RECORD && i < n) {
                     it[T2_PF * (size_t)n + i] = pf;
                     it[T2_PS * (size_t)n + i] = ps;
@@ -494,7 +492,7 @@ struct D2Engine {
                 Qss[r] = sgn2 * q.cd * rss;
                 // RMSProp + clamp, per coordinate
                 {
-                    const float vn = fmaf(C.rms_rho, vvf, (1.0f - C.rms_rho) * (pf * pf));
+                    const float vn = C.rms_rho * vvf + (1.0f - C.rms_rho) * pf * pf;
                     const float qq = rsqrtf(vn + C.rms_eps);
                     const float hm = (ok && unf > -kD2Clamp && unf < kD2Clamp) ? 1.0f : 0.0f;
                     const float hbb = hbf[r] * hm;
@@ -504,7 +502,7 @@ struct D2Engine {
                     hbf[r] = hbb;
                 }
                 {
-                    const float vn = fmaf(C.rms_rho, vvs, (1.0f - C.rms_rho) * (ps * ps));
+                    const float vn = C.rms_rho * vvs + (1.0f - C.rms_rho) * ps * ps;
                     const float qq = rsqrtf(vn + C.rms_eps);
                     const float hm = (ok && uns > -kD2Clamp && uns < kD2Clamp) ? 1.0f : 0.0f;
                     const float hbb = hbs[r] * hm;
@@ -649,8 +647,8 @@ struct D2Engine {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 const int i = row(r);
-                mmf[r] = i < n ? sqf * (th.rho_f * slot[i]) : 0.f;  // the forward's means, rounded alike
-                mms[r] = i < n ? sqs * (th.rho_s * slot[n + i]) : 0.f;
+                mmf[r] = i < n ? sqf * th.rho_f * slot[i] : 0.f;
+                mms[r] = i < n ? sqs * th.rho_s * slot[n + i] : 0.f;
             }
             __syncthreads();
             // pass B: components = own rows; sum_j rt (a_j - psi kappa sq dd) with the constant
